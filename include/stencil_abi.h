@@ -1,0 +1,216 @@
+/*
+ * stencil_abi.h - C ABI of the MI355X (gfx950) iterative star-stencil sweep.
+ *
+ * This is the drop-in boundary for the hot path of mrfarstad/thesis:
+ *   host  main()            -> dispatch_kernels(d_u1, d_u2)
+ *                              src/main.cu:64-67, src/stencil_dispatch.cu:138-158
+ *   launch  get_kernel()<<<grid, block, smem>>>(d_u1, d_u2, kstart, kend)
+ *                              src/stencil_dispatch.cu:14-42,147-157
+ *   knobs   -D DIM DIMENSIONS RADIUS ITERATIONS BLOCK_X/Y/Z COARSEN_X SMEM
+ *              PADDED REGISTER COOP HEURISTIC SMEM_PAD
+ *                              Makefile:21-71, include/constants.h:1-78
+ * The compile-time knobs become fields of st_params; the kernel choice of
+ * get_kernel() becomes st_params.variant.  All functions return a status
+ * code instead of exiting the process (include/stencil_utils.h:113-124 and
+ * include/helper_cuda.h:891-930 in the reference abort).
+ *
+ * Conventions
+ *   - Fields are single contiguous arrays, x fastest:
+ *       idx = i + j*nx + k*nx*ny          (src/stencil_kernel_base.cu:14)
+ *   - Device pointers are plain HIP device addresses; `stream` is a
+ *     hipStream_t passed as void* (NULL = the null stream).
+ *   - Only interior cells (R <= i < n-R on every axis) are written.  The
+ *     R-thick boundary shell of both buffers is read-only for the whole run.
+ *   - The caller owns every buffer.  No entry point allocates device memory
+ *     on the sweep path; stencil_residual_norm and the mixed/batched entry
+ *     points use a small workspace owned by the library (allocated once per
+ *     device on first use).
+ *   - Calls are stream-ordered.  They are re-entrant for disjoint buffers
+ *     and not thread-safe on the same buffers.
+ */
+#ifndef THESIS_AMD_STENCIL_ABI_H
+#define THESIS_AMD_STENCIL_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum st_status {
+    ST_OK = 0,
+    ST_EINVAL = 1,        /* invalid knobs or shapes (check_early_exit analogue) */
+    ST_ELDS = 2,          /* LDS request above the 160 KiB per workgroup limit   */
+    ST_ELAUNCH = 3,       /* a kernel launch or HIP call failed                  */
+    ST_ENOMEM = 4,        /* workspace allocation failed                         */
+    ST_EUNSUPPORTED = 5,  /* valid knobs that this build does not implement      */
+    ST_EDEVICE = 6        /* no usable gfx950 device                             */
+} st_status;
+
+/* Operator.  THESIS is the reference update (src/stencils.cu:8-19):
+ *     u' = (sum of the 2*D*R axis neighbours) / (float)(2*D*R) - u_centre
+ * JACOBI is the classical weighted-neighbour Jacobi sweep:
+ *     u' = weight * (sum of the 2*D*R axis neighbours)
+ * Both sum the neighbours in the reference order: i-prev (d=R..1),
+ * i-next (d=1..R), j-prev, j-next, k-prev, k-next
+ * (src/stencil_accumulators.cu:6-16,46-74). */
+typedef enum st_op { ST_OP_THESIS = 0, ST_OP_JACOBI = 1 } st_op;
+
+/* Kernel family, replacing get_kernel() (src/stencil_dispatch.cu:14-42):
+ *   BASE          base_{2d,3d}, base_unroll_*      all operands from HBM/cache
+ *   SMEM          smem_*, smem_unroll_*            LDS tile, halo from global
+ *   SMEM_PADDED   smem_padded_*                    LDS tile with R halo
+ *   SMEM_REGISTER smem_register_*                  register-queue march along
+ *                                                  the slowest axis (z in 3D,
+ *                                                  y in 2D), cross-lane x
+ *   COOP          coop (src/stencil_kernel_coop.cu) persistent cooperative
+ *                                                  kernel, grid-synchronised,
+ *                                                  temporal blocking
+ *   AUTO          fastest family for the shape (library's own choice)      */
+typedef enum st_variant {
+    ST_V_BASE = 0,
+    ST_V_SMEM = 1,
+    ST_V_SMEM_PADDED = 2,
+    ST_V_SMEM_REGISTER = 3,
+    ST_V_COOP = 4,
+    ST_V_AUTO = 5
+} st_variant;
+
+/* Storage type of the fields.  F32 and F64 compute in the storage type.
+ * BF16 and F16 store 16-bit fields and compute in FP32; they are used only by
+ * stencil_dispatch_mixed. */
+typedef enum st_dtype { ST_F32 = 0, ST_F64 = 1, ST_BF16 = 2, ST_F16 = 3 } st_dtype;
+
+/* Initial conditions written by stencil_init. */
+typedef enum st_init {
+    /* initialize_host_region (include/stencil_initializer.h:6-52): the
+     * outermost layer of u1 is 1.0, every other cell of u1 is 0; u2 is zeroed
+     * (the reference leaves it uninitialised; fresh memory is zero). */
+    ST_INIT_REFERENCE = 0,
+    /* every cell of the R-thick boundary shell of both buffers gets
+     * U(seed, idx) = (splitmix64(seed ^ idx*0x9E3779B97F4A7C15) >> 40) * 2^-24,
+     * the interior of both buffers is 0 */
+    ST_INIT_SEEDED = 1
+} st_init;
+
+typedef struct st_params {
+    int nx, ny, nz;       /* extents; the reference's DIM gives nx=ny=nz; nz is
+                             ignored (taken as 1) when dimensions == 2        */
+    int dimensions;       /* DIMENSIONS: 2 or 3                               */
+    int radius;           /* RADIUS >= 1                                       */
+    int iterations;       /* ITERATIONS >= 0                                   */
+    int block_x;          /* BLOCK_X/Y/Z: workgroup shape (0 = heuristic).     */
+    int block_y;          /*   SMEM_REGISTER: block_x is a multiple of 64 (one */
+    int block_z;          /*   wave across x), block_z is the z-chunk depth in */
+                          /*   planes (0 = whole axis split heuristically).    */
+                          /*   COOP: block_z is ignored.                       */
+    int coarsen_x;        /* COARSEN_X: cells per thread along x (>= 1)        */
+    int coarsen_y;        /* SMEM_REGISTER/COOP: rows per thread (0=heuristic) */
+    int smem_pad;         /* SMEM_PAD: extra LDS floats per tile row           */
+    int variant;          /* st_variant                                        */
+    int op;               /* st_op                                             */
+    int dtype;            /* st_dtype                                          */
+    int heuristic;        /* HEURISTIC: 1 = library picks block/coarsening     */
+    int temporal_steps;   /* COOP: sweeps fused per HBM pass (0 = heuristic)   */
+    int restart;          /* mixed precision: sweeps between defect
+                             corrections (0 = 10)                              */
+    int launch_per_pass;  /* COOP: 0 = one persistent cooperative launch with
+                             grid-wide barriers between passes, 1 = one launch
+                             per pass (kernel boundaries as the barrier)       */
+    double weight;        /* JACOBI neighbour weight (0 = 1/(2*D*R)); rounded
+                             to the storage type                               */
+} st_params;
+
+/* Resolved launch configuration (what get_kernel/set_block_dims/set_smem
+ * compute in src/stencil_dispatch.cu:46-122). */
+typedef struct st_launch_info {
+    int variant;                 /* resolved st_variant (AUTO replaced)        */
+    int block_x, block_y, block_z;
+    int grid_x, grid_y, grid_z;
+    int coarsen_x, coarsen_y;
+    int zchunk;                  /* planes per workgroup (SMEM_REGISTER)       */
+    int temporal_steps;          /* sweeps per pass (COOP)                     */
+    int lds_bytes;               /* LDS per workgroup                          */
+    int launches;                /* kernel launches for `iterations` sweeps    */
+} st_launch_info;
+
+/* Default knobs of include/constants.h:1-52: DIM 256, DIMENSIONS 3,
+ * RADIUS 1, ITERATIONS 64, BLOCK 32x2x2, COARSEN_X 4, base kernel,
+ * THESIS operator, FP32. */
+void stencil_params_default(st_params *p);
+
+/* Host-only validation, the analogue of check_early_exit
+ * (include/stencil_utils.h:113-124): SMEM_PADDED rejects block_x, block_y
+ * (and block_z in 3D) below RADIUS; SMEM_REGISTER rejects block_x (and
+ * block_y in 3D) below RADIUS.  Also rejects non-positive extents, extents not
+ * larger than 2*RADIUS, more than 1024 threads, and LDS above 160 KiB. */
+st_status stencil_validate(const st_params *p);
+
+/* Resolve the launch configuration without launching anything. */
+st_status stencil_launch_config(const st_params *p, st_launch_info *info);
+
+/* One sweep d_out <- S(d_in): one get_kernel()<<<...>>> launch
+ * (src/stencil_dispatch.cu:154).  d_in and d_out must not alias.
+ * Not available for the COOP variant (which fuses sweeps). */
+st_status stencil_sweep(const st_params *p, const void *d_in, void *d_out, void *stream);
+
+/* dispatch_kernels (src/stencil_dispatch.cu:138-158): p->iterations sweeps
+ * ping-ponging between d_u1 and d_u2.  On return *result_in_u2 (if not NULL)
+ * is 1 when the final field is in d_u2 and 0 when it is in d_u1; as in the
+ * reference it is d_u1 for an even iteration count.  The COOP variant
+ * schedules its fused passes so that the result lands in the same buffer as
+ * the per-sweep variants; the interior of the other buffer then holds an
+ * intermediate state, not sweep iterations-1.  If elapsed_ms is not NULL the
+ * call brackets the sweep loop with HIP events on `stream`, waits for the
+ * stop event and stores the time in milliseconds (host set-up excluded). */
+st_status stencil_dispatch(const st_params *p, void *d_u1, void *d_u2, void *stream,
+                           int *result_in_u2, float *elapsed_ms);
+
+/* Write the initial condition `kind` (st_init) into d_u1 and d_u2 (both
+ * nx*ny*nz elements of p->dtype, F32 or F64). */
+st_status stencil_init(const st_params *p, int kind, uint64_t seed, void *d_u1, void *d_u2,
+                       void *stream);
+
+/* FP64 residual norm ||weight*S(u) - u||_2 over the interior of the F32 or
+ * F64 field d_u (JACOBI operator), reduced on the GPU in FP64.  Blocks until
+ * the result is on the host. */
+st_status stencil_residual_norm(const st_params *p, const void *d_u, double *norm,
+                                void *stream);
+
+/* Mixed precision (p->dtype BF16 or F16, JACOBI operator, 3D or 2D):
+ * p->iterations sweeps of u = x + e where x is an FP32 base field holding the
+ * Dirichlet boundary, e a 16-bit correction and d a 16-bit defect.  Every
+ * p->restart sweeps the correction is folded into x in FP32 and the defect
+ * d = G(x) - x is recomputed in FP32.  Each correction sweep is
+ *     e' = round16(widen(d) + weight * sum of widen(e) neighbours)
+ * d_x (FP32, in/out), d_e1, d_e2, d_d (16-bit, nx*ny*nz each, scratch) and
+ * d_u (FP32 out, the final x + e).  Bit-exact with oracle/stencil_oracle.c. */
+st_status stencil_dispatch_mixed(const st_params *p, void *d_x, void *d_e1, void *d_e2,
+                                 void *d_d, void *d_u, void *stream, float *elapsed_ms);
+
+/* Human-readable status and the last error detail recorded on this thread. */
+const char *stencil_status_str(st_status s);
+const char *stencil_last_error(void);
+
+/* ------------------------------------------------------------------ */
+/* Host utilities for callers without their own HIP runtime plumbing    */
+/* (the CLI, tests, bench).  Thin wrappers over hipMalloc and friends.  */
+/* ------------------------------------------------------------------ */
+st_status st_device_count(int *count);
+st_status st_device_name(int device, char *buf, int len);
+st_status st_malloc(void **ptr, uint64_t bytes);
+st_status st_free(void *ptr);
+st_status st_memcpy_htod(void *dst, const void *src, uint64_t bytes, void *stream);
+st_status st_memcpy_dtoh(void *dst, const void *src, uint64_t bytes, void *stream);
+st_status st_memcpy_dtod(void *dst, const void *src, uint64_t bytes, void *stream);
+st_status st_memset(void *dst, int value, uint64_t bytes, void *stream);
+st_status st_stream_create(void **stream);
+st_status st_stream_destroy(void *stream);
+st_status st_stream_sync(void *stream);
+st_status st_device_sync(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* THESIS_AMD_STENCIL_ABI_H */

@@ -1,0 +1,109 @@
+"""Host-side checks of the C ABI that need no GPU: the library loads, it
+exports every function include/stencil_abi.h declares, and knob validation
+(the check_early_exit analogue) accepts and rejects what it should."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(\w+)\s*\([^;{]*\)\s*;", text, flags=re.M):
+            if not m.group(0).lstrip().startswith("typedef"):
+                names.append(m.group(1))
+    return names
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for must in ("stencil_dispatch", "stencil_sweep", "stencil_validate", "stencil_launch_config",
+                 "stencil_init", "stencil_residual_norm", "stencil_dispatch_mixed",
+                 "stencil_status_str", "stencil_params_default"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol(st):
+    L = C.CDLL(st.library_path())
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_params_layout_matches_header(st):
+    # st_params: 19 ints then a double -> 88 bytes with alignment
+    assert C.sizeof(st.StencilParams) == 88
+    p = st.StencilParams.default()
+    assert (p.nx, p.ny, p.nz, p.dimensions, p.radius, p.iterations) == (256, 256, 256, 3, 1, 64)
+    assert (p.block_x, p.block_y, p.block_z, p.coarsen_x) == (32, 2, 2, 4)
+    assert p.variant == st.V_BASE and p.op == st.OP_THESIS and p.dtype == st.F32
+
+
+def test_status_strings(st):
+    L = st.lib()
+    assert L.stencil_status_str(0) == b"ok"
+    assert L.stencil_status_str(2) == b"LDS budget exceeded"
+
+
+def _raises(st, status, **kw):
+    with pytest.raises(st.StencilError) as e:
+        st.check_early_exit(st.StencilParams.make(**kw))
+    assert e.value.status == status, str(e.value)
+
+
+def test_check_early_exit_rules(st):
+    # include/stencil_utils.h:113-124: padded needs bx, by (bz in 3D) >= R;
+    # register needs bx (by in 3D) >= R.
+    ok = dict(dim=64, radius=4, iterations=2)
+    _raises(st, 1, variant="smem_padded", block_x=2, block_y=8, block_z=8, **ok)
+    _raises(st, 1, variant="smem_padded", block_x=8, block_y=2, block_z=8, **ok)
+    _raises(st, 1, variant="smem_padded", block_x=8, block_y=8, block_z=2, **ok)
+    st.check_early_exit(st.StencilParams.make(variant="smem_padded", block_x=8, block_y=8,
+                                              block_z=2, dimensions=2, **ok))
+    _raises(st, 1, variant="smem_register", block_x=2, block_y=8, block_z=8, **ok)
+    _raises(st, 1, variant="smem_register", block_x=64, block_y=2, block_z=8, **ok)
+    st.check_early_exit(st.StencilParams.make(variant="smem_register", block_x=64, block_y=4,
+                                              block_z=16, **ok))
+    # plain smem and base have no radius rule
+    st.check_early_exit(st.StencilParams.make(variant="smem", block_x=2, block_y=2, block_z=2, **ok))
+
+
+def test_invalid_knobs(st):
+    _raises(st, 1, dim=2, radius=1)               # extent must exceed 2R
+    _raises(st, 1, dimensions=4)
+    _raises(st, 1, radius=0)
+    _raises(st, 1, iterations=-1)
+    _raises(st, 1, block_x=64, block_y=32, block_z=1, variant="base")  # 2048 threads
+    _raises(st, 1, dtype=st.BF16)                # 16-bit storage only through the mixed path
+    _raises(st, 2, variant="smem_padded", block_x=16, block_y=8, block_z=8, radius=8, dim=64,
+            coarsen_x=4)                         # LDS over 160 KiB
+    _raises(st, 5, variant="smem_register", radius=9, dim=64, heuristic=1)
+    _raises(st, 5, variant="coop", radius=5, dim=64, heuristic=1)
+    _raises(st, 5, variant="coop", dimensions=2, dim=64, heuristic=1)
+
+
+def test_launch_config_shapes(st):
+    base = st.launch_config(st.StencilParams.make(dim=512, radius=1, iterations=100, variant="base",
+                                                  block_x=32, block_y=8, block_z=4, coarsen_x=1))
+    assert (base["grid_x"], base["grid_y"], base["grid_z"]) == (16, 64, 128)  # 512/32, 512/8, 512/4
+    assert base["launches"] == 100 and base["lds_bytes"] == 0
+    pad = st.launch_config(st.StencilParams.make(dim=512, radius=1, iterations=100, variant="smem_padded",
+                                                 block_x=32, block_y=8, block_z=4, coarsen_x=1))
+    assert pad["lds_bytes"] == (32 + 2) * (8 + 2) * (4 + 2) * 4  # src/stencil_dispatch.cu:57
+    sm = st.launch_config(st.StencilParams.make(dim=512, radius=1, iterations=100, variant="smem",
+                                                block_x=32, block_y=8, block_z=4, coarsen_x=4))
+    assert sm["lds_bytes"] == 32 * 4 * 8 * 4 * 4 and sm["grid_x"] == 4
+    reg = st.launch_config(st.StencilParams.make(dim=512, radius=1, iterations=100,
+                                                 variant="smem_register", heuristic=1))
+    assert reg["block_x"] % 64 == 0 and reg["lds_bytes"] == 0 and reg["zchunk"] >= 16
+    coop = st.launch_config(st.StencilParams.make(dim=512, radius=1, iterations=100, variant="coop",
+                                                  heuristic=1, op=st.OP_JACOBI))
+    assert coop["temporal_steps"] >= 1 and coop["lds_bytes"] <= 160 * 1024
+    assert coop["launches"] == 1

@@ -1,0 +1,225 @@
+// 16-bit field storage (BF16 or FP16) with FP32 arithmetic and FP32 defect
+// correction, for the JACOBI operator.  No reference counterpart; the
+// arithmetic is restated bit for bit in oracle/stencil_oracle.c
+// (or_mixed_run) and checked against it.
+//
+// The field is u = x + e: x an FP32 base field that carries the Dirichlet
+// shell, e a 16-bit correction and d a 16-bit defect d = G(x) - x.  Because
+// the sweep G is affine, G(x + e) = G(x) + G0(e), so one sweep of u is the
+// correction sweep  e' = d + G0(e)  which moves 2 + 2 + 2 bytes per cell.
+// Every `restart` sweeps the correction is folded into x in FP32 and the
+// defect is recomputed in FP32 from the new x (the FP32-residual defect
+// correction); 16-bit rounding then only ever applies to increments.
+#include "st_device.hpp"
+#include "st_internal.hpp"
+
+namespace st {
+
+// Round-to-nearest-even FP32 -> BF16 / FP16 and back, in integer
+// arithmetic identical to the oracle's, so results are bit-exact.
+__device__ __forceinline__ uint16_t to_bf16(float f)
+{
+    uint32_t x = __float_as_uint(f);
+    if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40);
+    const uint32_t lsb = (x >> 16) & 1u;
+    x += 0x7fffu + lsb;
+    return (uint16_t)(x >> 16);
+}
+
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ uint16_t to_f16(float f)
+{
+    const uint32_t x = __float_as_uint(f);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const uint32_t ax = x & 0x7fffffffu;
+    if (ax > 0x7f800000u) return (uint16_t)(sign | 0x7e00u);
+    if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);
+    if (ax < 0x38800000u) {
+        if (ax < 0x33000000u) return (uint16_t)sign;
+        const uint32_t e = ax >> 23;
+        const uint32_t m = (ax & 0x7fffffu) | 0x800000u;
+        const uint32_t shift = 126u - e;
+        uint32_t q = m >> shift;
+        const uint32_t rem = m & ((1u << shift) - 1u);
+        const uint32_t half = 1u << (shift - 1u);
+        if (rem > half || (rem == half && (q & 1u))) q++;
+        return (uint16_t)(sign | q);
+    }
+    const uint32_t e = (ax >> 23) - 112u;
+    const uint32_t m = ax & 0x7fffffu;
+    uint32_t q = (e << 10) | (m >> 13);
+    const uint32_t rem = m & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) q++;
+    return (uint16_t)(sign | q);
+}
+
+__device__ __forceinline__ float from_f16(uint16_t h)
+{
+    const uint32_t sign = ((uint32_t)h & 0x8000u) << 16;
+    const uint32_t e = (h >> 10) & 0x1fu;
+    const uint32_t m = h & 0x3ffu;
+    uint32_t x;
+    if (e == 0) {
+        if (m == 0) x = sign;
+        else x = __float_as_uint((float)m * (1.0f / 16777216.0f)) | sign;
+    } else if (e == 31) {
+        x = sign | 0x7f800000u | (m << 13);
+    } else {
+        x = sign | ((e + 112u) << 23) | (m << 13);
+    }
+    return __uint_as_float(x);
+}
+
+template <bool F16> __device__ __forceinline__ uint16_t rnd16(float f) { return F16 ? to_f16(f) : to_bf16(f); }
+template <bool F16> __device__ __forceinline__ float wid16(uint16_t h) { return F16 ? from_f16(h) : from_bf16(h); }
+
+struct MixArgs {
+    Geom g;
+    int dims;
+    float w;
+};
+
+// Decompose interior index t into (i, j, k).
+__device__ __forceinline__ long long interior_idx(const MixArgs &a, long long t)
+{
+    const Geom &g = a.g;
+    const int R = g.r;
+    const int wx = g.nx - 2 * R, wy = g.ny - 2 * R;
+    const long long wxy = (long long)wx * wy;
+    const int kk = (int)(t / wxy);
+    const long long rem = t - (long long)kk * wxy;
+    const int j = R + (int)(rem / wx);
+    const int i = R + (int)(rem - (long long)(j - R) * wx);
+    const int k = a.dims == 3 ? R + kk : 0;
+    return i + j * g.sy + k * g.sz;
+}
+
+__device__ __forceinline__ long long interior_count(const MixArgs &a)
+{
+    const Geom &g = a.g;
+    const int R = g.r;
+    return (long long)(g.nx - 2 * R) * (g.ny - 2 * R) * (a.dims == 3 ? g.nz - 2 * R : 1);
+}
+
+// d = round16(G(x) - x) on the interior; G is the FP32 JACOBI cell.
+template <bool F16>
+__global__ __launch_bounds__(256) void k_mx_defect(const float *__restrict__ x,
+                                                   uint16_t *__restrict__ d, MixArgs a)
+{
+    const Geom &g = a.g;
+    const int R = g.r;
+    const long long n = interior_count(a), stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += stride) {
+        const long long idx = interior_idx(a, t);
+        float u = 0.0f;
+        for (int q = R; q >= 1; --q) u += x[idx - q];
+        for (int q = 1; q <= R; ++q) u += x[idx + q];
+        for (int q = R; q >= 1; --q) u += x[idx - q * g.sy];
+        for (int q = 1; q <= R; ++q) u += x[idx + q * g.sy];
+        if (a.dims == 3) {
+            for (int q = R; q >= 1; --q) u += x[idx - q * g.sz];
+            for (int q = 1; q <= R; ++q) u += x[idx + q * g.sz];
+        }
+        const float tcell = u * a.w;
+        d[idx] = rnd16<F16>(tcell - x[idx]);
+    }
+}
+
+// e' = round16(widen(d) + w * sum widen(e neighbours)).
+template <bool F16>
+__global__ __launch_bounds__(256) void k_mx_sweep(const uint16_t *__restrict__ e,
+                                                  const uint16_t *__restrict__ d,
+                                                  uint16_t *__restrict__ eo, MixArgs a)
+{
+    const Geom &g = a.g;
+    const int R = g.r;
+    const long long n = interior_count(a), stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += stride) {
+        const long long idx = interior_idx(a, t);
+        float s = 0.0f;
+        for (int q = R; q >= 1; --q) s += wid16<F16>(e[idx - q]);
+        for (int q = 1; q <= R; ++q) s += wid16<F16>(e[idx + q]);
+        for (int q = R; q >= 1; --q) s += wid16<F16>(e[idx - q * g.sy]);
+        for (int q = 1; q <= R; ++q) s += wid16<F16>(e[idx + q * g.sy]);
+        if (a.dims == 3) {
+            for (int q = R; q >= 1; --q) s += wid16<F16>(e[idx - q * g.sz]);
+            for (int q = 1; q <= R; ++q) s += wid16<F16>(e[idx + q * g.sz]);
+        }
+        const float sw = s * a.w;
+        eo[idx] = rnd16<F16>(wid16<F16>(d[idx]) + sw);
+    }
+}
+
+// x += widen(e); e <- 0 (whole grid, elementwise).
+template <bool F16>
+__global__ __launch_bounds__(256) void k_mx_fold(float *__restrict__ x, uint16_t *__restrict__ e,
+                                                 long long n)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += stride) {
+        x[i] = x[i] + wid16<F16>(e[i]);
+        e[i] = 0;
+    }
+}
+
+// u = x + widen(e).
+template <bool F16>
+__global__ __launch_bounds__(256) void k_mx_final(const float *__restrict__ x,
+                                                  const uint16_t *__restrict__ e,
+                                                  float *__restrict__ u, long long n)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += stride)
+        u[i] = x[i] + wid16<F16>(e[i]);
+}
+
+template <bool F16>
+static st_status mixed_impl(const Cfg &c, float *x, uint16_t *e1, uint16_t *e2, uint16_t *d,
+                            float *u, hipStream_t s)
+{
+    MixArgs a;
+    a.g.nx = c.nx; a.g.ny = c.ny; a.g.nz = c.nz; a.g.r = c.r;
+    a.g.sy = c.nx; a.g.sz = (long long)c.nx * c.ny;
+    a.dims = c.dims;
+    a.w = (float)c.weight;
+    const long long n = (long long)c.nx * c.ny * c.nz;
+    const int R = c.r;
+    const long long ni = (long long)(c.nx - 2 * R) * (c.ny - 2 * R) * (c.dims == 3 ? c.nz - 2 * R : 1);
+    auto blocks = [](long long cnt) {
+        long long b = (cnt + 255) / 256;
+        if (b > 32768) b = 32768;
+        return dim3((unsigned)(b < 1 ? 1 : b));
+    };
+    hipError_t err;
+    if ((err = hipMemsetAsync(e1, 0, n * 2, s)) != hipSuccess) return hip_check(err, "memset e1");
+    if ((err = hipMemsetAsync(e2, 0, n * 2, s)) != hipSuccess) return hip_check(err, "memset e2");
+    if ((err = hipMemsetAsync(d, 0, n * 2, s)) != hipSuccess) return hip_check(err, "memset d");
+    hipLaunchKernelGGL(k_mx_defect<F16>, blocks(ni), dim3(256), 0, s, x, d, a);
+    uint16_t *ea = e1, *eb = e2;
+    for (int it = 1; it <= c.iters; ++it) {
+        hipLaunchKernelGGL(k_mx_sweep<F16>, blocks(ni), dim3(256), 0, s, ea, d, eb, a);
+        uint16_t *t = ea; ea = eb; eb = t;
+        if (it % c.restart == 0 && it < c.iters) {
+            // eb's interior is overwritten by the next sweep and its shell is
+            // still 0, so only the live correction needs clearing.
+            hipLaunchKernelGGL(k_mx_fold<F16>, blocks(n), dim3(256), 0, s, x, ea, n);
+            hipLaunchKernelGGL(k_mx_defect<F16>, blocks(ni), dim3(256), 0, s, x, d, a);
+        }
+    }
+    hipLaunchKernelGGL(k_mx_final<F16>, blocks(n), dim3(256), 0, s, x, ea, u, n);
+    return hip_check(hipGetLastError(), "mixed launch");
+}
+
+st_status run_mixed(const Cfg &c, void *x, void *e1, void *e2, void *d, void *u, hipStream_t s)
+{
+    if (c.dtype == ST_F16)
+        return mixed_impl<true>(c, static_cast<float *>(x), static_cast<uint16_t *>(e1),
+                                static_cast<uint16_t *>(e2), static_cast<uint16_t *>(d),
+                                static_cast<float *>(u), s);
+    return mixed_impl<false>(c, static_cast<float *>(x), static_cast<uint16_t *>(e1),
+                             static_cast<uint16_t *>(e2), static_cast<uint16_t *>(d),
+                             static_cast<float *>(u), s);
+}
+
+}  // namespace st

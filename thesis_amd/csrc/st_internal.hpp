@@ -1,0 +1,64 @@
+// Library-internal host declarations: the resolved launch configuration and
+// the per-family launchers implemented in the .hip files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/stencil_abi.h"
+
+namespace st {
+
+// Knobs after validation and heuristics (the runtime form of the
+// reference's compile-time constants, include/constants.h:1-78).
+struct Cfg {
+    int nx, ny, nz;      // nz == 1 in 2D
+    int dims, r, iters;
+    int op, dtype, variant;
+    int bx, by, bz;      // workgroup shape as launched
+    int cx, cy;          // coarsening along x (cells per lane) and rows per lane
+    int pad;             // LDS row padding (floats)
+    int zchunk;          // planes per workgroup (register march)
+    int tsteps;          // sweeps per pass (cooperative temporal blocking)
+    int gx, gy, gz;      // grid
+    int lds;             // dynamic LDS bytes
+    double weight;       // JACOBI weight (resolved)
+    int restart;         // mixed precision defect-correction interval
+    int heuristic;       // HEURISTIC knob
+    int per_pass;        // COOP: one launch per pass instead of a cooperative launch
+};
+
+// Record a detail string for stencil_last_error() and return `s`.
+st_status fail(st_status s, const char *fmt, ...);
+
+st_status hip_check(hipError_t e, const char *what);
+
+// Per-family launch-shape resolution (fills block/grid/lds in `c`).
+st_status resolve_base(Cfg &c);
+st_status resolve_smem(Cfg &c);
+st_status resolve_padded(Cfg &c);
+st_status resolve_march(Cfg &c);
+st_status resolve_coop(Cfg &c);
+
+// One sweep in -> out.
+st_status launch_base(const Cfg &c, const void *in, void *out, hipStream_t s);
+st_status launch_smem(const Cfg &c, const void *in, void *out, hipStream_t s);
+st_status launch_padded(const Cfg &c, const void *in, void *out, hipStream_t s);
+st_status launch_march(const Cfg &c, const void *in, void *out, hipStream_t s);
+
+// Cooperative temporal-blocked run of c.iters sweeps; *in_u2 receives the
+// buffer that holds the result.
+st_status run_coop(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2);
+
+st_status launch_init(const Cfg &c, int kind, unsigned long long seed, void *u1, void *u2,
+                      hipStream_t s);
+st_status residual_norm(const Cfg &c, const void *u, double *norm, hipStream_t s);
+st_status run_mixed(const Cfg &c, void *x, void *e1, void *e2, void *d, void *u,
+                    hipStream_t s);
+
+// Library-owned device workspace (grows on demand, one per device).
+st_status workspace(size_t bytes, void **ptr);
+
+constexpr int kMaxLds = 160 * 1024;
+constexpr int kMaxThreads = 1024;
+
+}  // namespace st

@@ -1,0 +1,291 @@
+// COOP family: persistent cooperative kernel with grid-synchronised temporal
+// blocking.
+//
+// Replaces coop (src/stencil_kernel_coop.cu:5-49), whose launch the
+// reference comments out (src/stencil_dispatch.cu:160-177), and its
+// grid-stride "all ITERATIONS in one kernel with grid.sync() per sweep"
+// design.  Here one HBM pass advances the field by tb sweeps:
+//
+//   * a workgroup owns an output tile TX x TY (x, y) and a chunk of planes,
+//     and marches along z;
+//   * level 0 (the pass input) and every intermediate level t < tb keep a
+//     ring of 2R+1 planes in LDS; level t covers the tile grown by R*(tb-t)
+//     on each side in x and y (overlapped tiling, no exchange between
+//     workgroups), and level tb is written to HBM;
+//   * per input plane: one coalesced load of the level-0 region, then tb
+//     LDS-to-LDS sweeps of one plane each, one barrier per level;
+//   * passes are separated by a grid-wide barrier (cooperative launch) or
+//     by kernel boundaries (one launch per pass).
+//
+// Boundary cells (the R-thick shell) of level t take their value from the
+// buffer the per-sweep ping-pong would read at sweep s+t: u1 for even,
+// u2 for odd global sweep index, exactly as the reference's alternating
+// buffers provide it (both buffers' shells are never written).  The number
+// of passes is chosen with the parity of the iteration count, so the result
+// lands in the same buffer as with per-sweep launches.
+#include <hip/hip_cooperative_groups.h>
+
+#include "st_device.hpp"
+#include "st_internal.hpp"
+
+namespace st {
+
+struct CoopArgs {
+    Geom g;
+    int TX, TY;          // output tile
+    int zchunk;          // output planes per tile
+    int tiles_x, tiles_y, ntiles;
+    int iters;           // total sweeps
+    int npass;           // passes
+    int pass0, pass1;    // passes run by this launch
+    int lds_t;           // level count the LDS layout was sized for
+};
+
+__device__ __forceinline__ void pass_range(int iters, int npass, int p, int &s, int &tb)
+{
+    const int q = iters / npass, rem = iters % npass;
+    s = p * q + min(p, rem);
+    tb = q + (p < rem ? 1 : 0);
+}
+
+template <int OP, int R, typename T>
+__global__ __launch_bounds__(256) void k_coop(T *u1, T *u2, CoopArgs a, T kc)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T *lds = reinterpret_cast<T *>(lds_raw);
+    constexpr int RING = 2 * R + 1;
+    const Geom &g = a.g;
+    const int nthr = blockDim.x;
+    const int tid = threadIdx.x;
+
+    for (int p = a.pass0; p < a.pass1; ++p) {
+        int s, tb;
+        pass_range(a.iters, a.npass, p, s, tb);
+        const T *X = (p & 1) ? u2 : u1;  // pass input
+        T *Y = (p & 1) ? u1 : u2;        // pass output
+        // level frame: level-0 region of this pass
+        const int W0 = a.TX + 2 * R * tb;
+        const int H0 = a.TY + 2 * R * tb;
+        const int P0 = W0 * H0;  // LDS plane size (elements)
+
+        for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+            const int txi = tile % a.tiles_x;
+            const int rest = tile / a.tiles_x;
+            const int tyi = rest % a.tiles_y;
+            const int tzi = rest / a.tiles_y;
+            const int X0 = txi * a.TX, Y0 = tyi * a.TY;
+            const int z0 = tzi * a.zchunk, z1 = min(z0 + a.zchunk, g.nz);
+            const int fx = X0 - R * tb, fy = Y0 - R * tb;  // frame origin
+            const int kin_lo = max(z0 - R * tb, 0), kin_hi = min(z1 + R * tb, g.nz);
+
+            for (int kin = kin_lo; kin < kin_hi; ++kin) {
+                // ---- level 0: plane kin of the pass input (shell from B[s%2])
+                {
+                    const T *B = (s & 1) ? u2 : u1;
+                    T *dst = lds + (kin % RING) * P0;
+                    const bool kint = interior1(kin, g.nz, R);
+                    const long long pz = (long long)kin * g.sz;
+                    for (int e = tid; e < P0; e += nthr) {
+                        const int ly = e / W0, lx = e - ly * W0;
+                        const int x = fx + lx, y = fy + ly;
+                        if (x < 0 || x >= g.nx || y < 0 || y >= g.ny) continue;
+                        const long long idx = pz + (long long)y * g.sy + x;
+                        const bool in_ = kint && interior1(x, g.nx, R) && interior1(y, g.ny, R);
+                        dst[e] = in_ ? X[idx] : B[idx];
+                    }
+                }
+                __syncthreads();
+                // ---- levels 1..tb
+                for (int t = 1; t <= tb; ++t) {
+                    const int kt = kin - t * R;
+                    const int grow = R * (tb - t);
+                    const int lo = max(z0 - grow, 0), hi = min(z1 + grow, g.nz);
+                    if (kt >= lo && kt < hi) {
+                        const T *Bt = ((s + t) & 1) ? u2 : u1;
+                        const T *src = lds + (t - 1) * RING * P0;
+                        T *dst = lds + t * RING * P0 + (kt % RING) * P0;
+                        const bool kint = interior1(kt, g.nz, R);
+                        const long long pz = (long long)kt * g.sz;
+                        // region of level t inside the frame
+                        const int rx0 = R * t, ry0 = R * t;
+                        const int rw = a.TX + 2 * grow, rh = a.TY + 2 * grow;
+                        const int n = rw * rh;
+                        const T *sc = src + (kt % RING) * P0;
+                        for (int e = tid; e < n; e += nthr) {
+                            const int ry = e / rw, rx = e - ry * rw;
+                            const int lx = rx0 + rx, ly = ry0 + ry;
+                            const int x = fx + lx, y = fy + ly;
+                            if (x < 0 || x >= g.nx || y < 0 || y >= g.ny) continue;
+                            const long long idx = pz + (long long)y * g.sy + x;
+                            const int li = ly * W0 + lx;
+                            T v;
+                            if (kint && interior1(x, g.nx, R) && interior1(y, g.ny, R)) {
+                                T u = 0;
+#pragma unroll
+                                for (int d = R; d >= 1; --d) u += sc[li - d];
+#pragma unroll
+                                for (int d = 1; d <= R; ++d) u += sc[li + d];
+#pragma unroll
+                                for (int d = R; d >= 1; --d) u += sc[li - d * W0];
+#pragma unroll
+                                for (int d = 1; d <= R; ++d) u += sc[li + d * W0];
+#pragma unroll
+                                for (int d = R; d >= 1; --d) u += src[((kt - d) % RING) * P0 + li];
+#pragma unroll
+                                for (int d = 1; d <= R; ++d) u += src[((kt + d) % RING) * P0 + li];
+                                v = finish<OP>(u, sc[li], kc);
+                                if (t == tb) {
+                                    Y[idx] = v;
+                                    continue;
+                                }
+                            } else {
+                                if (t == tb) continue;  // shell is never written
+                                v = Bt[idx];
+                            }
+                            dst[li] = v;
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+        if (p + 1 < a.pass1) cooperative_groups::this_grid().sync();
+    }
+}
+
+// ---------------------------------------------------------------- host side
+
+template <int OP, int R, typename T>
+static const void *coop_fn()
+{
+    return reinterpret_cast<const void *>(&k_coop<OP, R, T>);
+}
+
+static const void *pick_coop(const Cfg &c)
+{
+    const bool f64 = c.dtype == ST_F64;
+#define CF(RR)                                                                    \
+    case RR:                                                                      \
+        if (f64) return c.op == OP_THESIS ? coop_fn<OP_THESIS, RR, double>()      \
+                                          : coop_fn<OP_JACOBI, RR, double>();     \
+        return c.op == OP_THESIS ? coop_fn<OP_THESIS, RR, float>()                \
+                                 : coop_fn<OP_JACOBI, RR, float>();
+    switch (c.r) {
+        CF(1)
+        CF(2)
+        CF(3)
+        CF(4)
+    default: return nullptr;
+    }
+#undef CF
+}
+
+static int coop_npass(int iters, int tmax)
+{
+    if (iters <= 0) return 0;
+    int n = (iters + tmax - 1) / tmax;
+    if ((n & 1) != (iters & 1)) ++n;
+    return n;
+}
+
+static long long coop_lds(const Cfg &c, int t)
+{
+    const int es = c.dtype == ST_F64 ? 8 : 4;
+    const long long w0 = c.bx + 2LL * c.r * t, h0 = c.by + 2LL * c.r * t;
+    return (long long)t * (2 * c.r + 1) * w0 * h0 * es;
+}
+
+// For COOP, block_x/block_y are the output tile (TX, TY), block_z the plane
+// chunk, coarsen_y is unused and the workgroup is 256 threads.
+st_status resolve_coop(Cfg &c)
+{
+    if (c.dims != 3) return fail(ST_EUNSUPPORTED, "COOP is implemented for 3D grids");
+    if (c.r > 4) return fail(ST_EUNSUPPORTED, "COOP is built for radius 1..4");
+    const bool heur = c.heuristic || c.bx <= 0;
+    if (heur) {
+        c.bx = 64;
+        c.by = 32;
+        c.bz = 0;
+    }
+    if (c.bx < 1 || c.by < 1) return fail(ST_EINVAL, "COOP tile must be positive");
+    int t = c.tsteps > 0 ? c.tsteps : 2;
+    if (t > c.iters && c.iters > 0) t = c.iters;
+    if (t < 1) t = 1;
+    while (t > 1 && coop_lds(c, t) > kMaxLds) --t;
+    if (coop_lds(c, t) > kMaxLds)
+        return fail(ST_ELDS, "COOP tile %dx%d needs %lld bytes of LDS", c.bx, c.by, coop_lds(c, t));
+    c.tsteps = t;
+    const int npass = coop_npass(c.iters, t);
+    // the largest pass length actually used
+    const int tmax_used = npass ? (c.iters + npass - 1) / npass : 1;
+    c.lds = (int)coop_lds(c, tmax_used > 0 ? tmax_used : 1);
+    const int tiles_x = (c.nx + c.bx - 1) / c.bx, tiles_y = (c.ny + c.by - 1) / c.by;
+    int chunk = c.bz > 0 ? c.bz : 0;
+    if (chunk <= 0) {
+        const long long per = (long long)tiles_x * tiles_y;
+        long long chunks = (1024 + per - 1) / per;
+        chunk = (int)((c.nz + chunks - 1) / chunks);
+        if (chunk < 32) chunk = 32;
+        if (chunk > c.nz) chunk = c.nz;
+    }
+    c.zchunk = chunk;
+    c.bz = chunk;
+    c.cx = 1;
+    c.cy = 1;
+    c.gy = c.gz = 1;
+    c.gx = 0;  // decided at launch from occupancy
+    return ST_OK;
+}
+
+st_status run_coop(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2)
+{
+    const void *fn = pick_coop(c);
+    if (!fn) return fail(ST_EUNSUPPORTED, "no COOP kernel for radius %d", c.r);
+    CoopArgs a;
+    a.g.nx = c.nx; a.g.ny = c.ny; a.g.nz = c.nz; a.g.r = c.r;
+    a.g.sy = c.nx; a.g.sz = (long long)c.nx * c.ny;
+    a.TX = c.bx; a.TY = c.by; a.zchunk = c.zchunk;
+    a.tiles_x = (c.nx + c.bx - 1) / c.bx;
+    a.tiles_y = (c.ny + c.by - 1) / c.by;
+    a.ntiles = a.tiles_x * a.tiles_y * ((c.nz + c.zchunk - 1) / c.zchunk);
+    a.iters = c.iters;
+    a.npass = coop_npass(c.iters, c.tsteps);
+    a.lds_t = c.tsteps;
+    *in_u2 = a.npass & 1;
+    if (a.npass == 0) return ST_OK;
+    const int threads = 256;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds);
+    if (e != hipSuccess) return hip_check(e, "hipFuncSetAttribute(COOP LDS)");
+    int per_cu = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, c.lds);
+    if (e != hipSuccess) return hip_check(e, "occupancy query");
+    if (per_cu < 1) return fail(ST_ELDS, "COOP kernel does not fit on a CU (lds %d)", c.lds);
+    int dev = 0, cus = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    int grid = per_cu * cus;
+    if (grid > a.ntiles) grid = a.ntiles;
+    const double kcd = c.op == OP_THESIS ? (double)(2 * c.dims * c.r) : c.weight;
+    float kf = (float)kcd;
+    double kd = kcd;
+    void *karg = c.dtype == ST_F64 ? (void *)&kd : (void *)&kf;
+    const bool coop = !c.per_pass;
+    if (coop) {
+        a.pass0 = 0;
+        a.pass1 = a.npass;
+        void *args[] = {&u1, &u2, &a, karg};
+        e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, c.lds, s);
+        if (e != hipSuccess) return hip_check(e, "hipLaunchCooperativeKernel");
+    } else {
+        for (int p = 0; p < a.npass; ++p) {
+            a.pass0 = p;
+            a.pass1 = p + 1;
+            void *args[] = {&u1, &u2, &a, karg};
+            e = hipLaunchKernel(fn, dim3(grid), dim3(threads), args, c.lds, s);
+            if (e != hipSuccess) return hip_check(e, "COOP pass launch");
+        }
+    }
+    return ST_OK;
+}
+
+}  // namespace st
