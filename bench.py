@@ -69,6 +69,22 @@ def load_traffic(tag: str):
     return None
 
 
+def probe(st, np):
+    d1 = st.DeviceArray((N, N, N), np.float32)
+    d2 = st.DeviceArray((N, N, N), np.float32)
+    p = st.StencilParams.make(dim=N, radius=RADIUS, iterations=2, op=st.OP_JACOBI, heuristic=1)
+    st.initialize_region(p, d1, d2, st.INIT_SEEDED, seed=SEED)
+    st.copy_f32(d2, d1, N ** 3)
+    st.synchronize()
+    print(f"[probe] copy {N}^3 fp32 ({4 * N ** 3} bytes read, {4 * N ** 3} written)", flush=True)
+    for v in ("smem_register", "base", "smem_padded", "coop"):
+        q = p.copy(variant=v)
+        st.initialize_region(q, d1, d2, st.INIT_SEEDED, seed=SEED)
+        st.dispatch_kernels(q, d1, d2)
+        st.synchronize()
+        print(f"[probe] {v}: 2 sweeps, launch {st.launch_config(q)}", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +96,9 @@ def main():
     ap.add_argument("--coarsen", default="", help="cx,cy override")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--probe", action="store_true",
+                    help="short run for counter collection: one streaming copy of a 512^3 "
+                         "field, then 2 sweeps of each kernel family")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -94,6 +113,10 @@ def main():
     if st.device_count() < 1:
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
+
+    if args.probe:
+        probe(st, np)
+        return
 
     kw = dict(dim=N, radius=RADIUS, iterations=SWEEPS, op=st.OP_JACOBI, dtype=st.F32,
               variant=args.variant, heuristic=1, temporal_steps=args.temporal_steps)

@@ -208,6 +208,9 @@ st_status st_stream_create(void **stream);
 st_status st_stream_destroy(void *stream);
 st_status st_stream_sync(void *stream);
 st_status st_device_sync(void);
+/* Streaming copy of n floats (n a multiple of 4, 16-byte aligned buffers):
+ * the bandwidth probe that calibrates HBM counters against the sweeps. */
+st_status st_copy_f32(void *dst, const void *src, uint64_t n, void *stream);
 
 #ifdef __cplusplus
 }

@@ -158,12 +158,13 @@ def lib():
         L.st_stream_destroy.argtypes = [P]
         L.st_stream_sync.argtypes = [P]
         L.st_device_sync.argtypes = []
+        L.st_copy_f32.argtypes = [P, P, U64, P]
         for name in ("stencil_validate", "stencil_launch_config", "stencil_sweep",
                      "stencil_dispatch", "stencil_init", "stencil_residual_norm",
                      "stencil_dispatch_mixed", "st_device_count", "st_device_name", "st_malloc",
                      "st_free", "st_memcpy_htod", "st_memcpy_dtoh", "st_memcpy_dtod",
                      "st_memset", "st_stream_create", "st_stream_destroy", "st_stream_sync",
-                     "st_device_sync"):
+                     "st_device_sync", "st_copy_f32"):
             getattr(L, name).restype = I
         _lib = L
     return _lib
@@ -188,6 +189,11 @@ def device_name(device: int = 0) -> str:
 
 def synchronize():
     _check(lib().st_device_sync())
+
+
+def copy_f32(dst, src, n: int, stream=None):
+    """Streaming-copy probe kernel (n floats)."""
+    _check(lib().st_copy_f32(_ptr(dst), _ptr(src), n, _sh(stream)))
 
 
 class Stream:

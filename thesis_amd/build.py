@@ -66,8 +66,16 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the `stencil` CLI (process contract of the reference's main())
+    os.makedirs(os.path.join(HERE, "bin"), exist_ok=True)
+    cli = os.path.join(HERE, "bin", "stencil")
+    cmd = [HIPCC] + COMMON + ["-o", cli, os.path.join(CSRC, "cli.cpp"), "-L" + HERE,
+                              "-lthesis_stencil", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"CLI build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {cli}")
     return LIB
 
 

@@ -143,6 +143,7 @@ static st_status resolve(const st_params *p, Cfg &c, bool mixed)
     c.pad = p->smem_pad;
     c.tsteps = p->temporal_steps;
     c.weight = p->weight > 0 ? p->weight : 1.0 / (2.0 * c.dims * c.r);
+    c.weight64 = c.weight;
     if (c.dtype == ST_F32 || c.dtype == ST_BF16 || c.dtype == ST_F16) c.weight = (double)(float)c.weight;
     c.restart = p->restart > 0 ? p->restart : 10;
     c.heuristic = p->heuristic;

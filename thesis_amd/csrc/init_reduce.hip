@@ -137,10 +137,10 @@ st_status residual_norm(const Cfg &c, const void *u, double *norm, hipStream_t s
     g.sy = c.nx; g.sz = (long long)c.nx * c.ny;
     if (c.dtype == ST_F64)
         hipLaunchKernelGGL(k_resid_partial<double>, dim3(kRedBlocks), dim3(256), 0, s,
-                           static_cast<const double *>(u), g, c.dims, c.weight, partial);
+                           static_cast<const double *>(u), g, c.dims, c.weight64, partial);
     else
         hipLaunchKernelGGL(k_resid_partial<float>, dim3(kRedBlocks), dim3(256), 0, s,
-                           static_cast<const float *>(u), g, c.dims, c.weight, partial);
+                           static_cast<const float *>(u), g, c.dims, c.weight64, partial);
     hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(256), 0, s, partial, kRedBlocks,
                        partial + kRedBlocks);
     st = hip_check(hipGetLastError(), "residual launch");
@@ -155,4 +155,33 @@ st_status residual_norm(const Cfg &c, const void *u, double *norm, hipStream_t s
     return ST_OK;
 }
 
+// ---------------------------------------------------------------- copy probe
+
+// Plain streaming copy (16-byte vectors).  Used to calibrate HBM counters and
+// to measure the attainable copy bandwidth beside the sweep kernels.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_copy_f32(f32x4 *__restrict__ dst,
+                                                  const f32x4 *__restrict__ src, long long n4)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4; i += stride)
+        __builtin_nontemporal_store(src[i], dst + i);
+}
+
 }  // namespace st
+
+extern "C" st_status st_copy_f32(void *dst, const void *src, uint64_t n, void *stream)
+{
+    using namespace st;
+    if (n % 4 || ((uintptr_t)dst | (uintptr_t)src) % 16)
+        return fail(ST_EINVAL, "copy probe needs a multiple of 4 elements and 16-byte alignment");
+    const long long n4 = (long long)(n / 4);
+    long long blocks = (n4 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_copy_f32, dim3((unsigned)blocks), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), static_cast<f32x4 *>(dst),
+                       static_cast<const f32x4 *>(src), n4);
+    return hip_check(hipGetLastError(), "copy launch");
+}

@@ -21,7 +21,8 @@ struct Cfg {
     int tsteps;          // sweeps per pass (cooperative temporal blocking)
     int gx, gy, gz;      // grid
     int lds;             // dynamic LDS bytes
-    double weight;       // JACOBI weight (resolved)
+    double weight;       // JACOBI weight (resolved, rounded to the storage type)
+    double weight64;     // the same weight unrounded (FP64 residual norm)
     int restart;         // mixed precision defect-correction interval
     int heuristic;       // HEURISTIC knob
     int per_pass;        // COOP: one launch per pass instead of a cooperative launch

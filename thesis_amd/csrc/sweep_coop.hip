@@ -76,11 +76,15 @@ __global__ __launch_bounds__(256) void k_coop(T *u1, T *u2, CoopArgs a, T kc)
             const int X0 = txi * a.TX, Y0 = tyi * a.TY;
             const int z0 = tzi * a.zchunk, z1 = min(z0 + a.zchunk, g.nz);
             const int fx = X0 - R * tb, fy = Y0 - R * tb;  // frame origin
-            const int kin_lo = max(z0 - R * tb, 0), kin_hi = min(z1 + R * tb, g.nz);
+            // Level t produces plane kin - t*R at step kin.  The march runs
+            // R*tb steps past the last input plane so that every level also
+            // reaches the top planes of its range; steps beyond nz-1 load
+            // nothing at level 0.
+            const int kin_lo = max(z0 - R * tb, 0), kin_hi = z1 + R * tb;
 
             for (int kin = kin_lo; kin < kin_hi; ++kin) {
                 // ---- level 0: plane kin of the pass input (shell from B[s%2])
-                {
+                if (kin < g.nz) {
                     const T *B = (s & 1) ? u2 : u1;
                     T *dst = lds + (kin % RING) * P0;
                     const bool kint = interior1(kin, g.nz, R);
@@ -202,15 +206,32 @@ st_status resolve_coop(Cfg &c)
     if (c.dims != 3) return fail(ST_EUNSUPPORTED, "COOP is implemented for 3D grids");
     if (c.r > 4) return fail(ST_EUNSUPPORTED, "COOP is built for radius 1..4");
     const bool heur = c.heuristic || c.bx <= 0;
+    int want = c.tsteps > 0 ? c.tsteps : 2;
+    if (want > c.iters && c.iters > 0) want = c.iters;
+    if (want < 1) want = 1;
     if (heur) {
-        c.bx = 64;
-        c.by = 32;
+        // First tile (largest first) that holds `want` levels in LDS; else
+        // the first that holds any.
+        static const int tiles[][2] = {{64, 32}, {64, 16}, {32, 32}, {32, 16}, {16, 16}, {16, 8}, {8, 8}};
+        int pick = -1, pick_t = 0;
+        for (int i = 0; i < 7; ++i) {
+            Cfg t = c;
+            t.bx = tiles[i][0];
+            t.by = tiles[i][1];
+            int tt = want;
+            while (tt > 1 && coop_lds(t, tt) > kMaxLds) --tt;
+            if (coop_lds(t, tt) > kMaxLds) continue;
+            if (tt == want) { pick = i; pick_t = tt; break; }
+            if (pick < 0) { pick = i; pick_t = tt; }
+        }
+        if (pick < 0) return fail(ST_ELDS, "no COOP tile fits in LDS");
+        c.bx = tiles[pick][0];
+        c.by = tiles[pick][1];
         c.bz = 0;
+        want = pick_t;
     }
     if (c.bx < 1 || c.by < 1) return fail(ST_EINVAL, "COOP tile must be positive");
-    int t = c.tsteps > 0 ? c.tsteps : 2;
-    if (t > c.iters && c.iters > 0) t = c.iters;
-    if (t < 1) t = 1;
+    int t = want;
     while (t > 1 && coop_lds(c, t) > kMaxLds) --t;
     if (coop_lds(c, t) > kMaxLds)
         return fail(ST_ELDS, "COOP tile %dx%d needs %lld bytes of LDS", c.bx, c.by, coop_lds(c, t));
@@ -261,8 +282,8 @@ st_status run_coop(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2)
     if (e != hipSuccess) return hip_check(e, "occupancy query");
     if (per_cu < 1) return fail(ST_ELDS, "COOP kernel does not fit on a CU (lds %d)", c.lds);
     int dev = 0, cus = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     int grid = per_cu * cus;
     if (grid > a.ntiles) grid = a.ntiles;
     const double kcd = c.op == OP_THESIS ? (double)(2 * c.dims * c.r) : c.weight;
