@@ -117,6 +117,10 @@ typedef struct st_params {
     int launch_per_pass;  /* COOP: 0 = one persistent cooperative launch with
                              grid-wide barriers between passes, 1 = one launch
                              per pass (kernel boundaries as the barrier)       */
+    int temporal_form;    /* COOP: 0 = library's choice, 1 = LDS-ring tiles
+                             (F32/F64, radius 1..4, any pass length),
+                             2 = register-queue full-row frames (F32, radius
+                             1, temporal_steps 2, even iteration count)        */
     double weight;        /* JACOBI neighbour weight (0 = 1/(2*D*R)); rounded
                              to the storage type                               */
 } st_params;

@@ -38,7 +38,7 @@ def test_library_exports_every_declared_symbol(st):
 
 
 def test_params_layout_matches_header(st):
-    # st_params: 19 ints then a double -> 88 bytes with alignment
+    # st_params: 20 ints then a double -> 88 bytes
     assert C.sizeof(st.StencilParams) == 88
     p = st.StencilParams.default()
     assert (p.nx, p.ny, p.nz, p.dimensions, p.radius, p.iterations) == (256, 256, 256, 3, 1, 64)

@@ -264,3 +264,45 @@ def test_full_size_512(gpu, oracle):
         results[v] = hashlib.sha256(h.tobytes()).hexdigest()
         assert h.min() >= 0.0 and h.max() < 1.0
     assert len(set(results.values())) == 1, results
+
+
+# register-queue temporal form (two fused sweeps per HBM pass)
+@pytest.mark.parametrize("shape,knobs,its", [
+    ((70, 37, 33), {}, 8),
+    ((130, 45, 37), {"coarsen_x": 4, "coarsen_y": 2, "block_z": 20}, 12),
+    ((1100, 20, 18), {"coarsen_x": 2, "coarsen_y": 4, "block_z": 18}, 4),  # frames overlap in x
+    ((64, 130, 70), {"coarsen_x": 2, "coarsen_y": 4, "block_z": 17}, 8),
+    ((512, 60, 40), {}, 4),
+], ids=str)
+@pytest.mark.parametrize("op", [0, 1])
+def test_coop_register_form(gpu, oracle, shape, knobs, its, op):
+    st = gpu
+    nx, ny, nz = shape
+    _announce("coop register form", shape, knobs, its, op)
+    if op == 0:
+        u1 = oracle.init_reference(nx, ny, nz, 3)
+        u2 = np.zeros_like(u1)
+    else:
+        u1 = oracle.init_seeded(nx, ny, nz, 3, 1, 0x5EED0004)
+        u2 = u1.copy()
+    want = oracle.run(u1.copy(), u2.copy(), 3, 1, op, its).copy()
+    heur = 0 if knobs else 1
+    for per_pass in (0, 1):
+        p = st.StencilParams.make(nx=nx, ny=ny, nz=nz, dimensions=3, radius=1, iterations=its,
+                                  op=op, variant="coop", temporal_form=2, temporal_steps=2,
+                                  heuristic=heur, launch_per_pass=per_pass,
+                                  **({"block_x": 64, "block_y": 16} if knobs else {}), **knobs)
+        cfg = st.launch_config(p)
+        assert cfg["temporal_steps"] == 2
+        got, in_u2 = run_gpu(st, p, u1, u2)
+        assert not in_u2
+        assert_bitwise(got, want, f"register form {shape} {knobs} per_pass={per_pass}")
+
+
+def test_coop_register_form_rejects_unsupported(gpu):
+    st = gpu
+    for kw in (dict(iterations=6), dict(radius=2, iterations=8), dict(dtype=st.F64, iterations=8)):
+        p = st.StencilParams.make(dim=40, variant="coop", temporal_form=2, temporal_steps=2,
+                                  heuristic=1, op=1, **kw)
+        with pytest.raises(st.StencilError):
+            st.check_early_exit(p)

@@ -59,7 +59,7 @@ class StencilParams(C.Structure):
         ("coarsen_x", C.c_int), ("coarsen_y", C.c_int), ("smem_pad", C.c_int),
         ("variant", C.c_int), ("op", C.c_int), ("dtype", C.c_int),
         ("heuristic", C.c_int), ("temporal_steps", C.c_int), ("restart", C.c_int),
-        ("launch_per_pass", C.c_int),
+        ("launch_per_pass", C.c_int), ("temporal_form", C.c_int),
         ("weight", C.c_double),
     ]
 

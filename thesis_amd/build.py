@@ -35,6 +35,7 @@ SOURCES = [
     "sweep_base.hip",
     "sweep_march.hip",
     "sweep_coop.hip",
+    "sweep_tblock.hip",
     "init_reduce.hip",
     "mixed.hip",
 ]

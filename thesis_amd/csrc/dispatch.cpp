@@ -100,7 +100,8 @@ static st_status validate_common(const st_params *p, bool mixed)
         return fail(ST_EINVAL, "sweeps store F32 or F64; BF16/F16 go through stencil_dispatch_mixed");
     }
     if (p->coarsen_x < 0 || p->coarsen_y < 0 || p->smem_pad < 0 || p->temporal_steps < 0 ||
-        p->launch_per_pass < 0 || p->launch_per_pass > 1)
+        p->launch_per_pass < 0 || p->launch_per_pass > 1 || p->temporal_form < 0 ||
+        p->temporal_form > 2)
         return fail(ST_EINVAL, "negative coarsening, padding or temporal steps");
     const bool heur = p->heuristic || p->block_x <= 0;
     if (!heur) {
@@ -148,6 +149,9 @@ static st_status resolve(const st_params *p, Cfg &c, bool mixed)
     c.restart = p->restart > 0 ? p->restart : 10;
     c.heuristic = p->heuristic;
     c.per_pass = p->launch_per_pass;
+    c.form = p->temporal_form;
+    c.kind = 0;
+    c.wx = c.wy = 0;
     c.zchunk = 0;
     c.lds = 0;
     c.gx = c.gy = c.gz = 1;
@@ -229,6 +233,7 @@ st_status stencil_launch_config(const st_params *p, st_launch_info *info)
         if (c.variant == ST_V_COOP) {
             int n = c.iters > 0 ? (c.iters + c.tsteps - 1) / c.tsteps : 0;
             if (n && (n & 1) != (c.iters & 1)) ++n;
+            if (c.kind == 2) n = c.iters / c.tsteps;
             info->launches = n ? (c.per_pass ? n : 1) : 0;
         } else {
             info->launches = c.iters;
@@ -288,11 +293,25 @@ done:
     return s;
 }
 
+// Initialisation and the residual depend on the geometry only: resolve them
+// with the kernel-shape knobs reset so that any sweep configuration works.
+static st_status resolve_geometry(const st_params *p, Cfg &c)
+{
+    if (!p) return fail(ST_EINVAL, "null params");
+    st_params q = *p;
+    q.variant = ST_V_BASE;
+    q.heuristic = 1;
+    q.block_x = q.block_y = q.block_z = 0;
+    q.coarsen_x = 1;
+    q.coarsen_y = 0;
+    return resolve(&q, c, false);
+}
+
 st_status stencil_init(const st_params *p, int kind, uint64_t seed, void *d_u1, void *d_u2,
                        void *stream)
 {
     Cfg c;
-    st_status s = resolve(p, c, false);
+    st_status s = resolve_geometry(p, c);
     if (s != ST_OK) return s;
     if (kind != ST_INIT_REFERENCE && kind != ST_INIT_SEEDED) return fail(ST_EINVAL, "unknown init kind");
     if (!d_u1 || !d_u2) return fail(ST_EINVAL, "null buffer");
@@ -302,7 +321,7 @@ st_status stencil_init(const st_params *p, int kind, uint64_t seed, void *d_u1, 
 st_status stencil_residual_norm(const st_params *p, const void *d_u, double *norm, void *stream)
 {
     Cfg c;
-    st_status s = resolve(p, c, false);
+    st_status s = resolve_geometry(p, c);
     if (s != ST_OK) return s;
     if (!d_u || !norm) return fail(ST_EINVAL, "null argument");
     return residual_norm(c, d_u, norm, static_cast<hipStream_t>(stream));

@@ -161,12 +161,22 @@ st_status residual_norm(const Cfg &c, const void *u, double *norm, hipStream_t s
 // to measure the attainable copy bandwidth beside the sweep kernels.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// Each thread moves 4 x 16 bytes with all loads issued before the stores.
 __global__ __launch_bounds__(256) void k_copy_f32(f32x4 *__restrict__ dst,
                                                   const f32x4 *__restrict__ src, long long n4)
 {
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4; i += stride)
-        __builtin_nontemporal_store(src[i], dst + i);
+    const long long base = (long long)blockIdx.x * 1024 + threadIdx.x;
+    f32x4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const long long i = base + u * 256;
+        if (i < n4) v[u] = __builtin_nontemporal_load(src + i);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const long long i = base + u * 256;
+        if (i < n4) __builtin_nontemporal_store(v[u], dst + i);
+    }
 }
 
 }  // namespace st
@@ -177,8 +187,8 @@ extern "C" st_status st_copy_f32(void *dst, const void *src, uint64_t n, void *s
     if (n % 4 || ((uintptr_t)dst | (uintptr_t)src) % 16)
         return fail(ST_EINVAL, "copy probe needs a multiple of 4 elements and 16-byte alignment");
     const long long n4 = (long long)(n / 4);
-    long long blocks = (n4 + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
+    long long blocks = (n4 + 1023) / 1024;
+    if (blocks > 0x7fffffffLL) return fail(ST_EINVAL, "copy probe too large");
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_copy_f32, dim3((unsigned)blocks), dim3(256), 0,
                        static_cast<hipStream_t>(stream), static_cast<f32x4 *>(dst),

@@ -26,6 +26,9 @@ struct Cfg {
     int restart;         // mixed precision defect-correction interval
     int heuristic;       // HEURISTIC knob
     int per_pass;        // COOP: one launch per pass instead of a cooperative launch
+    int form;            // COOP form requested (0 auto, 1 LDS ring, 2 register queues)
+    int kind;            // COOP form resolved
+    int wx, wy;          // COOP register form: waves across x and y
 };
 
 // Record a detail string for stencil_last_error() and return `s`.
@@ -49,6 +52,11 @@ st_status launch_march(const Cfg &c, const void *in, void *out, hipStream_t s);
 // Cooperative temporal-blocked run of c.iters sweeps; *in_u2 receives the
 // buffer that holds the result.
 st_status run_coop(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2);
+
+// Register-queue temporal blocking (sweep_tblock.hip).
+bool tblock_eligible(const Cfg &c, int t);
+st_status resolve_tblock(Cfg &c, int t);
+st_status run_tblock(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2);
 
 st_status launch_init(const Cfg &c, int kind, unsigned long long seed, void *u1, void *u2,
                       hipStream_t s);

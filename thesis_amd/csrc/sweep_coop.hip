@@ -95,7 +95,10 @@ __global__ __launch_bounds__(256) void k_coop(T *u1, T *u2, CoopArgs a, T kc)
                         if (x < 0 || x >= g.nx || y < 0 || y >= g.ny) continue;
                         const long long idx = pz + (long long)y * g.sy + x;
                         const bool in_ = kint && interior1(x, g.nx, R) && interior1(y, g.ny, R);
-                        dst[e] = in_ ? X[idx] : B[idx];
+                        // select the buffer, then load once (a select of two
+                        // loads would fetch both)
+                        const T *srcp = in_ ? X : B;
+                        dst[e] = srcp[idx];
                     }
                 }
                 __syncthreads();
@@ -206,6 +209,20 @@ st_status resolve_coop(Cfg &c)
     if (c.dims != 3) return fail(ST_EUNSUPPORTED, "COOP is implemented for 3D grids");
     if (c.r > 4) return fail(ST_EUNSUPPORTED, "COOP is built for radius 1..4");
     const bool heur = c.heuristic || c.bx <= 0;
+    {
+        // register-queue form: requested, or the library's choice for
+        // heuristic shapes where it applies
+        const int t = c.tsteps > 0 ? c.tsteps : 2;
+        if (c.form == 2 || (c.form == 0 && heur && tblock_eligible(c, t))) {
+            if (!tblock_eligible(c, t))
+                return fail(ST_EUNSUPPORTED, "register temporal form needs FP32, radius 1, "
+                                             "temporal_steps 2 and an even iteration count (T=%d)", t);
+            if (heur) { c.cx = 0; c.cy = 0; c.zchunk = 0; }
+            else c.zchunk = c.bz > 0 ? c.bz : 0;
+            return resolve_tblock(c, t);
+        }
+    }
+    c.kind = 1;
     int want = c.tsteps > 0 ? c.tsteps : 2;
     if (want > c.iters && c.iters > 0) want = c.iters;
     if (want < 1) want = 1;
@@ -260,6 +277,7 @@ st_status resolve_coop(Cfg &c)
 
 st_status run_coop(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2)
 {
+    if (c.kind == 2) return run_tblock(c, u1, u2, s, in_u2);
     const void *fn = pick_coop(c);
     if (!fn) return fail(ST_EUNSUPPORTED, "no COOP kernel for radius %d", c.r);
     CoopArgs a;
