@@ -69,6 +69,99 @@ def load_traffic(tag: str):
     return None
 
 
+def bench_batched(st, np, args):
+    """512 uneven 3D grids (edges log-uniform in [16, 128], radius 1 or 2),
+    50 JACOBI sweeps each: one hipGraph launch, one persistent cooperative
+    launch, and per-problem launches."""
+    from thesis_amd import workloads
+    from thesis_amd.tuner.batched_tiles import load_rules
+
+    shapes = workloads.batched_shapes(512)
+    its = 50
+    rules = load_rules()
+    stream = st.Stream()
+    probs, bufs = [], []
+    for i, (nx, ny, nz, r) in enumerate(shapes):
+        p = st.StencilParams.make(nx=nx, ny=ny, nz=nz, dimensions=3, radius=r, op=st.OP_JACOBI)
+        d1 = st.DeviceArray(p.shape, np.float32)
+        d2 = st.DeviceArray(p.shape, np.float32)
+        st.initialize_region(p, d1, d2, st.INIT_SEEDED, seed=0x5EED0004 + i, stream=stream)
+        probs.append(dict(nx=nx, ny=ny, nz=nz, radius=r, iterations=its, op=st.OP_JACOBI, u1=d1, u2=d2))
+        bufs += [d1, d2]
+    stream.sync()
+    cells = workloads.batched_interior_cells(shapes) * its
+    traffic = 4 * (workloads.batched_cells(shapes) + workloads.batched_interior_cells(shapes)) * its
+    res = {}
+    for name, mode in (("graph", st.BATCH_GRAPH), ("persistent", st.BATCH_PERSISTENT),
+                       ("sequential", st.BATCH_SEQUENTIAL)):
+        st.dispatch_batched(probs, rules, mode, stream=stream)  # warm-up
+        t = min(st.dispatch_batched(probs, rules, mode, stream=stream, timed=True)
+                for _ in range(max(1, args.steps)))
+        res[name] = {"ms": round(t, 3), "gcells_per_s": round(cells / t / 1e6, 2),
+                     "hbm_frac": round(traffic / (t / 1e3) / 8e12, 4)}
+    out = {"metric": "batched GCells/s (512 uneven grids, 50 sweeps)", "value": res["graph"]["gcells_per_s"],
+           "unit": "GCells/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
+           "data": "synthetic: 512 grids, edges round(16*8^U) per axis, radius 1|2, seeded boundaries",
+           "config": {"workload": "batched uneven 3D JACOBI, 50 sweeps", "total_cells": workloads.batched_cells(shapes),
+                      "tile_rules": rules},
+           "modes": res,
+           "speedup_graph_vs_sequential": round(res["sequential"]["ms"] / res["graph"]["ms"], 2),
+           "speedup_persistent_vs_sequential": round(res["sequential"]["ms"] / res["persistent"]["ms"], 2)}
+    print(json.dumps(out), flush=True)
+
+
+def bench_mixed(st, np, args):
+    """384^3, radius 4 (25-point), JACOBI c_r = 1/24, 200 sweeps: FP32, FP64 and
+    BF16/FP16 storage with FP32 defect correction every 10 sweeps.  Errors are
+    measured against the FP64 field on the GPU (whose bit-exactness with the CPU
+    FP64 oracle the parity tests establish)."""
+    n, r, its = 384, 4, 200
+    seed = 0x5EED0003
+    base = dict(dim=n, radius=r, iterations=its, op=st.OP_JACOBI, variant="auto", heuristic=1)
+    shape = (n, n, n)
+    interior = (n - 2 * r) ** 3
+    out = {"metric": "384^3 R=4 error vs FP64 oracle per storage type", "unit": "GCells/s",
+           "config": {"workload": "3D 25-point JACOBI, 384^3, 200 sweeps, restart 10"}, "results": {}}
+    p64 = st.StencilParams.make(dtype=st.F64, **base)
+    a = st.DeviceArray(shape, np.float64)
+    b = st.DeviceArray(shape, np.float64)
+    st.initialize_region(p64, a, b, st.INIT_SEEDED, seed=seed)
+    _, ms = st.dispatch_kernels(p64, a, b, timed=True)
+    ref = a.numpy()
+    a.free(); b.free()
+    out["results"]["f64"] = {"ms": round(ms, 3), "gcells_per_s": round(interior * its / ms / 1e6, 1),
+                             "max_abs": 0.0, "l2": 0.0}
+
+    def errs(u):
+        d = u.astype(np.float64) - ref
+        return float(np.max(np.abs(d))), float(np.sqrt(np.sum(d * d)) / np.sqrt(np.sum(ref * ref)))
+
+    p32 = st.StencilParams.make(dtype=st.F32, **base)
+    a = st.DeviceArray(shape, np.float32)
+    b = st.DeviceArray(shape, np.float32)
+    st.initialize_region(p32, a, b, st.INIT_SEEDED, seed=seed)
+    x0 = a.numpy()
+    _, ms = st.dispatch_kernels(p32, a, b, timed=True)
+    mx, l2 = errs(a.numpy())
+    out["results"]["f32"] = {"ms": round(ms, 3), "gcells_per_s": round(interior * its / ms / 1e6, 1),
+                             "max_abs": mx, "l2_rel": l2}
+    for name, dt in (("bf16", st.BF16), ("f16", st.F16)):
+        pm = st.StencilParams.make(dtype=dt, restart=10, **{k: v for k, v in base.items() if k != "variant"})
+        x = st.DeviceArray.from_numpy(x0)
+        e1 = st.DeviceArray(shape, np.uint16)
+        e2 = st.DeviceArray(shape, np.uint16)
+        d = st.DeviceArray(shape, np.uint16)
+        u = st.DeviceArray(shape, np.float32)
+        ms = st.dispatch_mixed(pm, x, e1, e2, d, u, timed=True)
+        mx, l2 = errs(u.numpy())
+        out["results"][name] = {"ms": round(ms, 3), "gcells_per_s": round(interior * its / ms / 1e6, 1),
+                                "max_abs": mx, "l2_rel": l2}
+        for t in (x, e1, e2, d, u):
+            t.free()
+    out["value"] = out["results"]["f32"]["gcells_per_s"]
+    print(json.dumps(out), flush=True)
+
+
 def probe(st, np):
     d1 = st.DeviceArray((N, N, N), np.float32)
     d2 = st.DeviceArray((N, N, N), np.float32)
@@ -96,6 +189,12 @@ def main():
     ap.add_argument("--coarsen", default="", help="cx,cy override")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--workload", default="sweep512",
+                    choices=["sweep512", "batched", "mixed384"],
+                    help="sweep512: the headline 512^3 sweep (default); batched: 512 uneven "
+                         "grids in one graph / persistent launch vs per-problem launches; "
+                         "mixed384: 384^3 R=4 in FP32, FP64 and 16-bit storage with errors "
+                         "against the FP64 oracle")
     ap.add_argument("--probe", action="store_true",
                     help="short run for counter collection: one streaming copy of a 512^3 "
                          "field, then 2 sweeps of each kernel family")
@@ -116,6 +215,12 @@ def main():
 
     if args.probe:
         probe(st, np)
+        return
+    if args.workload == "batched":
+        bench_batched(st, np, args)
+        return
+    if args.workload == "mixed384":
+        bench_mixed(st, np, args)
         return
 
     kw = dict(dim=N, radius=RADIUS, iterations=SWEEPS, op=st.OP_JACOBI, dtype=st.F32,

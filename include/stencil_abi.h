@@ -192,6 +192,44 @@ st_status stencil_residual_norm(const st_params *p, const void *d_u, double *nor
 st_status stencil_dispatch_mixed(const st_params *p, void *d_x, void *d_e1, void *d_e2,
                                  void *d_d, void *d_u, void *stream, float *elapsed_ms);
 
+/* ------------------------------------------------------------------ */
+/* Batched sweeps of many independent grids (3D, FP32, radius 1-2).     */
+/* ------------------------------------------------------------------ */
+
+/* One grid of a batch; the per-problem contract is stencil_dispatch's:
+ * `iterations` sweeps ping-ponging between u1 and u2, result in u1 for an
+ * even count, boundary shell of both buffers read-only. */
+typedef struct st_problem {
+    int nx, ny, nz;
+    int radius;
+    int iterations;
+    int op;               /* st_op */
+    double weight;        /* JACOBI weight, 0 = 1/(6*radius) */
+    void *u1, *u2;        /* device buffers, nx*ny*nz floats each */
+} st_problem;
+
+/* Autotuned tile shape for a class of problems: the first rule with the
+ * problem's radius and max_cells >= nx*ny*nz applies. */
+typedef struct st_tile_rule {
+    int radius;
+    long long max_cells;
+    int coarsen_y;        /* rows per wave: 1, 2 or 4 (radius 2: 1 or 2) */
+    int zchunk;           /* planes per wave tile (0 = whole z extent)    */
+} st_tile_rule;
+
+typedef enum st_batch_mode {
+    ST_BATCH_GRAPH = 0,       /* one hipGraph replaying a batched launch per sweep */
+    ST_BATCH_PERSISTENT = 1,  /* one cooperative launch, grid barrier per sweep    */
+    ST_BATCH_SEQUENTIAL = 2   /* stencil_dispatch per problem (baseline)           */
+} st_batch_mode;
+
+/* Advance every problem of the batch by its iteration count.  `stream`
+ * must not be the null stream for ST_BATCH_GRAPH (a private stream is used
+ * when it is NULL).  elapsed_ms (optional) covers the sweep launches only. */
+st_status stencil_dispatch_batched(const st_problem *problems, int nproblems,
+                                   const st_tile_rule *rules, int nrules, int mode,
+                                   void *stream, float *elapsed_ms);
+
 /* Human-readable status and the last error detail recorded on this thread. */
 const char *stencil_status_str(st_status s);
 const char *stencil_last_error(void);

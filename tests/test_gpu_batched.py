@@ -1,0 +1,91 @@
+"""Batched sweeps of uneven grids: every mode against the CPU oracle per
+problem (bit-exact), and the three modes against each other on the full
+512-problem batch."""
+import hashlib
+
+import numpy as np
+import pytest
+
+from thesis_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _alloc(st, oracle, shapes, its, ops, seed, stream):
+    probs, host, bufs = [], [], []
+    for i, (nx, ny, nz, r) in enumerate(shapes):
+        u1 = oracle.init_seeded(nx, ny, nz, 3, r, seed + i)
+        if ops[i] == 0:  # THESIS: random interior, buffer 2 shell zero
+            rng = np.random.default_rng(i)
+            u1 = np.where(u1 == 0, rng.random(u1.shape), u1).astype(np.float32)
+            u2 = np.zeros_like(u1)
+        else:
+            u2 = u1.copy()
+        d1 = st.DeviceArray.from_numpy(u1, stream)
+        d2 = st.DeviceArray.from_numpy(u2, stream)
+        probs.append(dict(nx=nx, ny=ny, nz=nz, radius=r, iterations=its[i], op=ops[i], u1=d1, u2=d2))
+        host.append((u1, u2))
+        bufs += [d1, d2]
+    return probs, host, bufs
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_batched_small_vs_oracle(gpu, oracle, mode):
+    st = gpu
+    stream = st.Stream()
+    shapes = workloads.batched_shapes(24, seed=77, lo=16, hi=48)
+    its = [4 + (i % 5) for i in range(24)]
+    ops = [i % 2 for i in range(24)]
+    print(f"[case] batched mode {mode}", flush=True)
+    probs, host, bufs = _alloc(st, oracle, shapes, its, ops, 1000, stream)
+    rules = [dict(radius=1, max_cells=20000, coarsen_y=4, zchunk=8),
+             dict(radius=2, max_cells=10 ** 9, coarsen_y=2, zchunk=0)]
+    st.dispatch_batched(probs, rules, mode, stream=stream)
+    stream.sync()
+    for i, (q, (u1, u2)) in enumerate(zip(probs, host)):
+        want = oracle.run(u1.copy(), u2.copy(), 3, q["radius"], q["op"], q["iterations"])
+        got = (q["u2"] if q["iterations"] % 2 else q["u1"]).numpy(stream)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (i, shapes[i], q["op"])
+
+
+def test_batched_full_modes_agree(gpu, oracle):
+    st = gpu
+    stream = st.Stream()
+    shapes = workloads.batched_shapes(512)
+    n = len(shapes)
+    assert 5e7 < workloads.batched_cells(shapes) < 2e8
+    its = [50] * n
+    ops = [1] * n
+    digests = {}
+    for mode in (0, 1, 2):
+        print(f"[case] full batch mode {mode}", flush=True)
+        probs = []
+        bufs = []
+        for i, (nx, ny, nz, r) in enumerate(shapes):
+            p = st.StencilParams.make(nx=nx, ny=ny, nz=nz, dimensions=3, radius=r, op=1)
+            d1 = st.DeviceArray(p.shape, np.float32)
+            d2 = st.DeviceArray(p.shape, np.float32)
+            st.initialize_region(p, d1, d2, st.INIT_SEEDED, seed=0x5EED0004 + i, stream=stream)
+            probs.append(dict(nx=nx, ny=ny, nz=nz, radius=r, iterations=50, op=1, u1=d1, u2=d2))
+            bufs += [d1, d2]
+        ms = st.dispatch_batched(probs, [], mode, stream=stream, timed=True)
+        stream.sync()
+        h = hashlib.sha256()
+        for q in probs:
+            h.update(q["u1"].numpy(stream).tobytes())
+        digests[mode] = h.hexdigest()
+        print(f"[case] mode {mode}: {ms:.2f} ms", flush=True)
+        for b in bufs:
+            b.free()
+    assert len(set(digests.values())) == 1, digests
+    # spot-check one problem of the batch against the oracle
+    nx, ny, nz, r = shapes[3]
+    u = oracle.init_seeded(nx, ny, nz, 3, r, 0x5EED0004 + 3)
+    want = oracle.run(u.copy(), u.copy(), 3, r, 1, 50)
+    p = st.StencilParams.make(nx=nx, ny=ny, nz=nz, dimensions=3, radius=r, op=1)
+    d1 = st.DeviceArray(p.shape, np.float32)
+    d2 = st.DeviceArray(p.shape, np.float32)
+    st.initialize_region(p, d1, d2, st.INIT_SEEDED, seed=0x5EED0004 + 3, stream=stream)
+    st.dispatch_batched([dict(nx=nx, ny=ny, nz=nz, radius=r, iterations=50, op=1, u1=d1, u2=d2)],
+                        [], 0, stream=stream)
+    assert np.array_equal(d1.numpy(stream).view(np.uint32), want.view(np.uint32))

@@ -114,6 +114,22 @@ class LaunchInfo(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class Problem(C.Structure):
+    """Mirror of ``st_problem``."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("radius", C.c_int),
+                ("iterations", C.c_int), ("op", C.c_int), ("weight", C.c_double),
+                ("u1", C.c_void_p), ("u2", C.c_void_p)]
+
+
+class TileRule(C.Structure):
+    """Mirror of ``st_tile_rule``."""
+    _fields_ = [("radius", C.c_int), ("max_cells", C.c_longlong), ("coarsen_y", C.c_int),
+                ("zchunk", C.c_int)]
+
+
+BATCH_GRAPH, BATCH_PERSISTENT, BATCH_SEQUENTIAL = 0, 1, 2
+
+
 class StencilError(RuntimeError):
     def __init__(self, status: int, detail: str):
         self.status = status
@@ -159,12 +175,15 @@ def lib():
         L.st_stream_sync.argtypes = [P]
         L.st_device_sync.argtypes = []
         L.st_copy_f32.argtypes = [P, P, U64, P]
+        L.stencil_dispatch_batched.argtypes = [C.POINTER(Problem), I, C.POINTER(TileRule), I, I, P,
+                                               C.POINTER(C.c_float)]
         for name in ("stencil_validate", "stencil_launch_config", "stencil_sweep",
                      "stencil_dispatch", "stencil_init", "stencil_residual_norm",
                      "stencil_dispatch_mixed", "st_device_count", "st_device_name", "st_malloc",
                      "st_free", "st_memcpy_htod", "st_memcpy_dtoh", "st_memcpy_dtod",
                      "st_memset", "st_stream_create", "st_stream_destroy", "st_stream_sync",
-                     "st_device_sync", "st_copy_f32"):
+                     "st_device_sync", "st_copy_f32",
+                     "stencil_dispatch_batched"):
             getattr(L, name).restype = I
         _lib = L
     return _lib
@@ -323,3 +342,31 @@ def dispatch_mixed(params: StencilParams, d_x, d_e1, d_e2, d_d, d_u, stream=None
     _check(lib().stencil_dispatch_mixed(_p(params), _ptr(d_x), _ptr(d_e1), _ptr(d_e2), _ptr(d_d),
                                         _ptr(d_u), _sh(stream), C.byref(ms) if timed else None))
     return ms.value if timed else None
+
+
+def dispatch_batched(problems, rules=(), mode: int = BATCH_GRAPH, stream=None, timed: bool = False):
+    """Advance a batch of independent grids.  `problems` is a sequence of
+    Problem (or dicts with its fields, buffers as DeviceArray or pointers);
+    `rules` a sequence of TileRule (or dicts)."""
+    n = len(problems)
+    parr = (Problem * max(n, 1))()
+    for i, q in enumerate(problems):
+        if isinstance(q, Problem):
+            parr[i] = q
+        else:
+            parr[i] = Problem(q["nx"], q["ny"], q["nz"], q.get("radius", 1), q.get("iterations", 1),
+                              q.get("op", OP_JACOBI), q.get("weight", 0.0), _vp(q["u1"]), _vp(q["u2"]))
+    rl = list(rules)
+    rarr = (TileRule * max(len(rl), 1))()
+    for i, r in enumerate(rl):
+        rarr[i] = r if isinstance(r, TileRule) else TileRule(r["radius"], r["max_cells"],
+                                                             r["coarsen_y"], r["zchunk"])
+    ms = C.c_float(0.0)
+    _check(lib().stencil_dispatch_batched(parr, n, rarr, len(rl), mode, _sh(stream),
+                                          C.byref(ms) if timed else None))
+    return ms.value if timed else None
+
+
+def _vp(x):
+    p = _ptr(x)
+    return p.value if isinstance(p, C.c_void_p) else p

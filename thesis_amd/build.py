@@ -36,6 +36,7 @@ SOURCES = [
     "sweep_march.hip",
     "sweep_coop.hip",
     "sweep_tblock.hip",
+    "batched.hip",
     "init_reduce.hip",
     "mixed.hip",
 ]

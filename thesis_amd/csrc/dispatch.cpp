@@ -44,30 +44,32 @@ st_status hip_check(hipError_t e, const char *what)
 // ---------------------------------------------------------------- workspace
 
 static std::mutex g_ws_mutex;
-static void *g_ws[64];
-static size_t g_ws_size[64];
+constexpr int kSlots = 4, kDevs = 64;
+static void *g_ws[kDevs][kSlots];
+static size_t g_ws_size[kDevs][kSlots];
 
-st_status workspace(size_t bytes, void **ptr)
+st_status workspace_slot(int slot, size_t bytes, void **ptr)
 {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return hip_check(e, "hipGetDevice");
-    if (dev < 0 || dev >= 64) return fail(ST_EDEVICE, "device index %d out of range", dev);
+    if (dev < 0 || dev >= kDevs) return fail(ST_EDEVICE, "device index %d out of range", dev);
+    if (slot < 0 || slot >= kSlots) return fail(ST_EINVAL, "workspace slot %d", slot);
     std::lock_guard<std::mutex> lock(g_ws_mutex);
-    if (g_ws_size[dev] < bytes) {
-        if (g_ws[dev]) {
+    if (g_ws_size[dev][slot] < bytes) {
+        if (g_ws[dev][slot]) {
             e = hipDeviceSynchronize();
             if (e != hipSuccess) return hip_check(e, "workspace sync");
-            (void)hipFree(g_ws[dev]);
-            g_ws[dev] = nullptr;
-            g_ws_size[dev] = 0;
+            (void)hipFree(g_ws[dev][slot]);
+            g_ws[dev][slot] = nullptr;
+            g_ws_size[dev][slot] = 0;
         }
         size_t sz = bytes < (1u << 20) ? (1u << 20) : bytes;
-        e = hipMalloc(&g_ws[dev], sz);
+        e = hipMalloc(&g_ws[dev][slot], sz);
         if (e != hipSuccess) return hip_check(e, "workspace hipMalloc");
-        g_ws_size[dev] = sz;
+        g_ws_size[dev][slot] = sz;
     }
-    *ptr = g_ws[dev];
+    *ptr = g_ws[dev][slot];
     return ST_OK;
 }
 
@@ -350,6 +352,27 @@ st_status stencil_dispatch_mixed(const st_params *p, void *d_x, void *d_e1, void
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+    }
+    return s;
+}
+
+st_status stencil_dispatch_batched(const st_problem *problems, int nproblems,
+                                   const st_tile_rule *rules, int nrules, int mode, void *stream,
+                                   float *elapsed_ms)
+{
+    if (mode < ST_BATCH_GRAPH || mode > ST_BATCH_SEQUENTIAL) return fail(ST_EINVAL, "unknown batch mode");
+    if (nrules < 0 || (nrules > 0 && !rules)) return fail(ST_EINVAL, "bad rule table");
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    hipStream_t own = nullptr;
+    if (!hs && mode == ST_BATCH_GRAPH) {
+        st_status s = hip_check(hipStreamCreate(&own), "stream create");
+        if (s != ST_OK) return s;
+        hs = own;
+    }
+    st_status s = run_batched(problems, nproblems, rules, nrules, mode, hs, elapsed_ms);
+    if (own) {
+        (void)hipStreamSynchronize(own);
+        (void)hipStreamDestroy(own);
     }
     return s;
 }

@@ -64,8 +64,12 @@ st_status residual_norm(const Cfg &c, const void *u, double *norm, hipStream_t s
 st_status run_mixed(const Cfg &c, void *x, void *e1, void *e2, void *d, void *u,
                     hipStream_t s);
 
-// Library-owned device workspace (grows on demand, one per device).
-st_status workspace(size_t bytes, void **ptr);
+// Library-owned device workspace (grows on demand, one per device and slot).
+st_status workspace_slot(int slot, size_t bytes, void **ptr);
+inline st_status workspace(size_t bytes, void **ptr) { return workspace_slot(0, bytes, ptr); }
+
+st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules, int nrules,
+                      int mode, hipStream_t s, float *elapsed_ms);
 
 constexpr int kMaxLds = 160 * 1024;
 constexpr int kMaxThreads = 1024;
