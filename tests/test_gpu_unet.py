@@ -1,0 +1,240 @@
+"""GPU U-Net kernels against the FP64 restatement (oracle/unet_oracle.py).
+
+Inputs are drawn, rounded to BF16 on the host and uploaded, so the oracle
+sees exactly the values the MFMA kernels multiply; the only remaining
+difference is FP32 accumulation order, bounded here by a relative error of
+1e-5 per layer.  The full network's gradients are compared with the
+BF16-mimicking FP64 network at a relative L2 error of 2e-3.  Training must
+lower the loss, and the trained network must beat its own input guess.
+"""
+import numpy as np
+import pytest
+
+from oracle import unet_oracle as U
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    """float array -> BF16 bit pattern (uint16) after round-to-nearest-even."""
+    return (U.bf16_round(a).astype(np.float32).view(np.uint32) >> 16).astype(np.uint16)
+
+
+def unbits(b):
+    return (b.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a - b)) / max(np.linalg.norm(np.ravel(b)), 1e-30))
+
+
+def _conv_case(st, nn, rng, shape, ci, co, relu, bias=True):
+    N, D, H, W = shape
+    x = U.bf16_round(rng.standard_normal(shape + (ci,)))
+    kp = nn.kpad(ci)
+    w = np.zeros((co, kp))
+    w[:, :27 * ci] = U.bf16_round(rng.standard_normal((co, 27 * ci)) * 0.2)
+    b = rng.standard_normal(co).astype(np.float32) if bias else None
+    dx = st.DeviceArray.from_numpy(bits(x).reshape(-1, ci))
+    dw = st.DeviceArray.from_numpy(bits(w))
+    db = st.DeviceArray.from_numpy(b) if bias else None
+    of = st.DeviceArray((N * D * H * W, co), np.float32)
+    ob = st.DeviceArray((N * D * H * W, co), np.uint16)
+    nn.conv3(dx, dw, db, ob, of, shape, ci, co, relu=relu)
+    want = U.conv3(x, w[:, :27 * ci], b.astype(np.float64) if bias else None)
+    if relu:
+        want = U.relu(want)
+    return x, w, of.numpy().reshape(want.shape), unbits(ob.numpy()).reshape(want.shape), want
+
+
+@pytest.mark.parametrize("shape,ci,co", [((2, 5, 6, 20), 16, 32), ((1, 4, 4, 64), 8, 16),
+                                         ((1, 6, 3, 9), 48, 16), ((1, 3, 4, 5), 96, 32),
+                                         ((2, 4, 4, 4), 64, 64), ((1, 3, 5, 7), 16, 48)])
+def test_conv3_forward(gpu, shape, ci, co):
+    from thesis_amd import nn
+    rng = np.random.default_rng(ci * 100 + co)
+    for relu in (False, True):
+        x, w, f32, b16, want = _conv_case(gpu, nn, rng, shape, ci, co, relu)
+        assert rel(f32, want) < 1e-5, (shape, ci, co, relu)
+        # the BF16 output is the FP32 result rounded: within one BF16 step
+        assert np.all(np.abs(b16 - f32) <= np.abs(f32) * 2 ** -8 + 1e-30)
+
+
+def test_conv3_backward_data_and_weights(gpu):
+    st = gpu
+    from thesis_amd import nn
+    rng = np.random.default_rng(7)
+    shape, ci, co = (2, 4, 5, 12), 16, 32
+    N, D, H, W = shape
+    P = N * D * H * W
+    x = U.bf16_round(rng.standard_normal(shape + (ci,)))
+    w = np.zeros((co, nn.kpad(ci)))
+    w[:, :27 * ci] = U.bf16_round(rng.standard_normal((co, 27 * ci)) * 0.2)
+    gy = U.bf16_round(rng.standard_normal(shape + (co,)))
+    gx_ref, gw_ref, gb_ref = U.conv3_bwd(x, w[:, :27 * ci], gy)
+    dx = st.DeviceArray.from_numpy(bits(x).reshape(P, ci))
+    dw = st.DeviceArray.from_numpy(bits(w))
+    dg = st.DeviceArray.from_numpy(bits(gy).reshape(P, co))
+    wt = st.DeviceArray((ci, nn.kpad(co)), np.uint16)
+    nn.wt_transpose(dw, wt, ci, co)
+    gx = st.DeviceArray((P, ci), np.float32)
+    nn.conv3(dg, wt, None, None, gx, shape, co, ci)
+    assert rel(gx.numpy().reshape(gx_ref.shape), gx_ref) < 1e-5
+    gw = st.DeviceArray((co, nn.kpad(ci)), np.float32)
+    gw.fill_zero()
+    nn.conv3_wgrad(dx, dg, gw, shape, ci, co)
+    g = gw.numpy()
+    assert rel(g[:, :27 * ci], gw_ref) < 1e-5
+    assert not g[:, 27 * ci:].any()  # padding columns get no gradient
+    gb = st.DeviceArray((co,), np.float32)
+    gb.fill_zero()
+    nn.bias_grad(dg, gb, P, co)
+    assert rel(gb.numpy(), gb_ref) < 1e-5
+
+
+def test_small_layers(gpu):
+    st = gpu
+    from thesis_amd import nn
+    rng = np.random.default_rng(9)
+    N, D, H, W, C = 2, 4, 6, 8, 16
+    x = U.bf16_round(rng.standard_normal((N, D, H, W, C)))
+    dx = st.DeviceArray.from_numpy(bits(x).reshape(-1, C))
+    # pool
+    dp = st.DeviceArray((N * D * H * W // 8, C), np.uint16)
+    nn.pool_fwd(dx, dp, (N, D // 2, H // 2, W // 2), C)
+    assert np.array_equal(unbits(dp.numpy()).reshape(N, D // 2, H // 2, W // 2, C), U.bf16_round(U.pool(x)))
+    g = rng.standard_normal((N, D // 2, H // 2, W // 2, C)).astype(np.float32)
+    gin = st.DeviceArray.from_numpy(np.zeros((N, D, H, W, C), np.float32))
+    nn.pool_bwd(st.DeviceArray.from_numpy(g), gin, (N, D // 2, H // 2, W // 2), C)
+    assert rel(gin.numpy(), U.pool_bwd(g.astype(np.float64))) < 1e-6
+    # upsample into a wider tensor, and back
+    xs = U.bf16_round(rng.standard_normal((N, 2, 3, 4, 8)))
+    dup = st.DeviceArray((N * 4 * 6 * 8, 24), np.uint16)
+    dup.fill_zero()
+    nn.up_fwd(st.DeviceArray.from_numpy(bits(xs).reshape(-1, 8)), dup, (N, 2, 3, 4), 8, 24, 16)
+    got = unbits(dup.numpy()).reshape(N, 4, 6, 8, 24)
+    assert np.array_equal(got[..., 16:], U.up(xs)) and not got[..., :16].any()
+    gw_ = rng.standard_normal((N, 4, 6, 8, 24)).astype(np.float32)
+    gsm = st.DeviceArray((N * 2 * 3 * 4, 8), np.float32)
+    nn.up_bwd(st.DeviceArray.from_numpy(gw_), gsm, (N, 2, 3, 4), 8, 24, 16)
+    assert rel(gsm.numpy().reshape(N, 2, 3, 4, 8), U.up_bwd(gw_[..., 16:].astype(np.float64))) < 1e-6
+    # head forward/backward
+    P = N * D * H * W
+    hw = rng.standard_normal(16).astype(np.float32)
+    hb = np.array([0.25], np.float32)
+    y = st.DeviceArray((P,), np.float32)
+    nn.head_fwd(dx, st.DeviceArray.from_numpy(hw), st.DeviceArray.from_numpy(hb), y, P, 16)
+    assert rel(y.numpy(), U.head(x.reshape(P, C), hw, hb[0])) < 1e-6
+    gy = rng.standard_normal(P).astype(np.float32)
+    gxh = st.DeviceArray((P, 16), np.float32)
+    dhw = st.DeviceArray.from_numpy(np.zeros(16, np.float32))
+    dhb = st.DeviceArray.from_numpy(np.zeros(1, np.float32))
+    nn.head_bwd(dx, st.DeviceArray.from_numpy(hw), st.DeviceArray.from_numpy(gy), gxh, dhw, dhb, P, 16)
+    assert rel(gxh.numpy(), gy[:, None].astype(np.float64) * hw) < 1e-6
+    assert rel(dhw.numpy(), gy.astype(np.float64) @ x.reshape(P, C)) < 1e-5
+    assert abs(dhb.numpy()[0] - gy.astype(np.float64).sum()) < 1e-3
+
+
+def test_features_trilinear_faces(gpu):
+    st = gpu
+    from thesis_amd import nn
+    from thesis_amd import warmstart as ws
+    n = 12
+    modes = ws.face_modes(2, 0x5EED0005)
+    u = st.DeviceArray((2, n, n, n), np.float32)
+    u.fill_zero()
+    nn.fourier_faces(u, st.DeviceArray.from_numpy(modes), 2, n, ws.MODES)
+    got = u.numpy()
+    for b in range(2):
+        assert np.max(np.abs(got[b] - U.fourier_field(modes[b].astype(np.float64), n))) < 1e-5
+    feat = st.DeviceArray((2 * n ** 3, 8), np.uint16)
+    base = st.DeviceArray((2, n, n, n), np.float32)
+    nn.features(u, feat, base, 2, n, 8)
+    f_ref, b_ref = U.features(got.astype(np.float64))
+    assert np.max(np.abs(base.numpy() - b_ref)) < 1e-5
+    assert np.max(np.abs(unbits(feat.numpy()).reshape(f_ref.shape) - f_ref)) <= 2 ** -8 * np.max(np.abs(f_ref))
+    m, M = 9, 17
+    c = np.random.default_rng(1).standard_normal((m, m, m)).astype(np.float32)
+    out = st.DeviceArray.from_numpy(np.zeros((M, M, M), np.float32))
+    nn.trilinear(st.DeviceArray.from_numpy(c), out, m, M)
+    want = U.trilinear(c.astype(np.float64), M)
+    o = out.numpy()
+    assert np.max(np.abs(o[1:-1, 1:-1, 1:-1] - want[1:-1, 1:-1, 1:-1])) < 1e-5
+    assert not o[0].any() and not o[:, :, -1].any()  # shell untouched
+
+
+def test_network_gradients(gpu):
+    st = gpu
+    from thesis_amd import warmstart as ws
+    n = 16
+    net = ws.UNet(1, n, seed=3)
+    modes = ws.face_modes(1, 0x5EED0006)
+    from thesis_amd import nn
+    net.u.fill_zero(net.s)
+    nn.fourier_faces(net.u, st.DeviceArray.from_numpy(modes), 1, n, ws.MODES, net.s)
+    # a synthetic target: the blend plus a smooth bump
+    u0 = net.u.numpy(net.s)
+    _, base0 = U.features(u0.astype(np.float64))
+    g = np.arange(n) / (n - 1)
+    bump = np.sin(np.pi * g)[:, None, None] * np.sin(np.pi * g)[None, :, None] * np.sin(np.pi * g)[None, None, :]
+    target = st.DeviceArray.from_numpy((base0[0] + 0.3 * bump).astype(np.float32))
+    net.forward(target=target)
+    net.backward()
+    gpu_grads = net.grads()
+    loss_gpu = net.loss_value()
+    # FP64 restatement with the network's BF16 weights and inputs
+    sd = net.state_dict()
+    params = {name: (U.bf16_round(sd[name + ".w"]), sd[name + ".b"].astype(np.float64))
+              for name, _, _ in ws.LAYERS}
+    params["head"] = (sd["head.w"].astype(np.float64), sd["head.b"].astype(np.float64))
+    ref = U.UNetRef(params, bf16=True)
+    # the GPU's own features and base (FP32 blend) as the oracle's inputs
+    x = unbits(net.a["x0"].numpy(net.s)).reshape(1, n, n, n, 8)
+    base = net.base.numpy(net.s).astype(np.float64).reshape(1, n, n, n)
+    y = ref.forward(x)[0]
+    interior = np.zeros((n, n, n), bool)
+    interior[1:-1, 1:-1, 1:-1] = True
+    tgt = (base0[0] + 0.3 * bump).astype(np.float32).astype(np.float64)
+    pred = np.where(interior, base[0] + y, base[0])
+    count = (n - 2) ** 3
+    loss_ref = np.sum(((pred - tgt) ** 2)[interior]) / count
+    assert abs(loss_gpu - loss_ref) < 1e-4 * loss_ref
+    gy = np.where(interior, 2.0 * (pred - tgt) / count, 0.0)[None]
+    G = ref.backward(gy)
+    worst = 0.0
+    for name, _, _ in ws.LAYERS:
+        e = rel(gpu_grads[name][0], G[name][0])
+        worst = max(worst, e)
+        assert e < 2e-3, (name, e)
+        assert rel(gpu_grads[name][1], G[name][1]) < 2e-3, name
+    assert rel(gpu_grads["head"][0], G["head"][0]) < 2e-3
+    print(f"[unet] worst weight-gradient relative error {worst:.2e}", flush=True)
+
+
+def test_training_and_warm_start_gain(gpu):
+    """Train briefly on method-generated 32^3 solves; the network must lower
+    its loss and predict held-out solutions better than its input guess."""
+    st = gpu
+    from thesis_amd import nn
+    from thesis_amd import warmstart as ws
+    n, count = 32, 64
+    stream = st.Stream()
+    fields, _ = ws.generate_solves(count, n, seed=1000, sweeps=3000, stream=stream)
+    net = ws.UNet(4, n, seed=5, stream=stream)
+    log = ws.train(net, fields, count, steps=150, lr=2e-3, lr_final=2e-4, log_every=25)
+    first, last = log[0][1], log[-1][1]
+    print(f"[unet] loss {first:.3e} -> {last:.3e}", flush=True)
+    assert last < 0.25 * first
+    test, _ = ws.generate_solves(4, n, seed=99000, sweeps=3000, stream=stream)
+    net.load_fields(test, range(4))
+    net.forward(target=test)
+    loss_net = net.loss_value()
+    pred = net.pred.numpy(stream)
+    base = net.base.numpy(stream)
+    t = test.numpy(stream).reshape(pred.shape)
+    interior = np.zeros((4, n, n, n), bool)
+    interior[:, 1:-1, 1:-1, 1:-1] = True
+    err_net = np.sqrt(np.mean((pred - t)[interior.reshape(pred.shape)] ** 2))
+    err_blend = np.sqrt(np.mean((base - t)[interior.reshape(pred.shape)] ** 2))
+    print(f"[unet] held-out rms error: network {err_net:.3e}, blend {err_blend:.3e}", flush=True)
+    assert err_net < 0.5 * err_blend

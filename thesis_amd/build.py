@@ -37,6 +37,7 @@ SOURCES = [
     "sweep_coop.hip",
     "sweep_tblock.hip",
     "batched.hip",
+    "unet.hip",
     "init_reduce.hip",
     "mixed.hip",
 ]
