@@ -59,6 +59,11 @@ st_status st_nn_head_bwd(const void *x, const float *w, const float *g, float *g
  * pred; with a target: loss += sum (u - t)^2 (FP64) and g = 2*scale*(u - t) */
 st_status st_nn_mse(const float *base, const float *y, const float *target, float *g, double *loss,
                     float *pred, int N, int n, double scale, void *stream);
+/* out = a*x + b*y over n floats */
+st_status st_nn_axpby(float *out, float a, const float *x, float b, const float *y, long long n,
+                      void *stream);
+/* loss += sum x^2 (FP64 accumulation); g = 2*scale*x when g is not NULL */
+st_status st_nn_sumsq(const float *x, float *g, double *loss, long long n, double scale, void *stream);
 /* Adam step (bias-corrected) with an optional BF16 copy of the weights */
 st_status st_nn_adam(float *w, const float *g, float *m, float *v, void *wbf, long long n, float lr,
                      float b1, float b2, float eps, int step, void *stream);

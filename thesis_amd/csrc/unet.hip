@@ -510,6 +510,36 @@ __global__ __launch_bounds__(256) void k_mse(const float *__restrict__ base,
     if (threadIdx.x == 0 && loss) atomicAdd(loss, red[0]);
 }
 
+// out = a*x + b*y (FP32)
+__global__ void k_axpby(float *__restrict__ out, float a, const float *__restrict__ x, float b,
+                        const float *__restrict__ y, long long n)
+{
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x)
+        out[t] = a * x[t] + b * y[t];
+}
+
+// loss += sum x^2 (FP64), g = 2*scale*x
+__global__ __launch_bounds__(256) void k_sumsq(const float *__restrict__ x, float *__restrict__ g,
+                                               double *__restrict__ loss, long long n, double scale)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x) {
+        const float v = x[t];
+        acc += (double)v * v;
+        if (g) g[t] = (float)(2.0 * scale) * v;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && loss) atomicAdd(loss, red[0]);
+}
+
 // Adam (Kingma & Ba 2015) with bias correction; writes the BF16 copy of the
 // weights for the MFMA kernels (zero columns of padded rows stay zero
 // because their gradients are zero).
@@ -802,6 +832,23 @@ st_status st_nn_adam(float *w, const float *g, float *m, float *v, void *wbf, lo
     hipLaunchKernelGGL(k_adam, gs_grid(n), dim3(256), 0, static_cast<hipStream_t>(stream), w, g, m, v,
                        static_cast<bf16_t *>(wbf), n, lr, b1, b2, eps, c1, c2);
     return hip_check(hipGetLastError(), "adam launch");
+}
+
+st_status st_nn_axpby(float *out, float a, const float *x, float b, const float *y, long long n,
+                      void *stream)
+{
+    hipLaunchKernelGGL(k_axpby, gs_grid(n), dim3(256), 0, static_cast<hipStream_t>(stream), out, a, x, b,
+                       y, n);
+    return hip_check(hipGetLastError(), "axpby launch");
+}
+
+st_status st_nn_sumsq(const float *x, float *g, double *loss, long long n, double scale, void *stream)
+{
+    long long blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_sumsq, dim3((unsigned)blocks), dim3(256), 0, static_cast<hipStream_t>(stream), x,
+                       g, loss, n, scale);
+    return hip_check(hipGetLastError(), "sumsq launch");
 }
 
 st_status st_nn_features(const float *u, void *feat, float *base, int N, int n, int C, void *stream)

@@ -38,6 +38,8 @@ def _bind():
         "st_nn_mse": ([P, P, P, P, P, P, I, I, D, P], I),
         "st_nn_adam": ([P, P, P, P, P, LL, F, F, F, F, I, P], I),
         "st_nn_features": ([P, P, P, I, I, I, P], I),
+        "st_nn_axpby": ([P, F, P, F, P, LL, P], I),
+        "st_nn_sumsq": ([P, P, P, LL, D, P], I),
         "st_nn_trilinear": ([P, P, I, I, P], I),
         "st_nn_fourier_faces": ([P, P, I, I, I, P], I),
     }
@@ -162,3 +164,13 @@ def trilinear(src, dst, m, M, stream=None):
 def fourier_faces(u, modes, N, n, M, stream=None):
     _bind()
     _check(lib().st_nn_fourier_faces(_p(u), _p(modes), N, n, M, _sh(stream)))
+
+
+def axpby(out, a, x, b, y, n, stream=None):
+    _bind()
+    _check(lib().st_nn_axpby(_p(out), a, _p(x), b, _p(y), n, _sh(stream)))
+
+
+def sumsq(x, g, loss, n, scale, stream=None):
+    _bind()
+    _check(lib().st_nn_sumsq(_p(x), _p(g), _p(loss), n, scale, _sh(stream)))

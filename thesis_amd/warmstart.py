@@ -139,6 +139,8 @@ class UNet:
         self.gA3, self.gB3 = f32(P3, 64), f32(P3, 64)
         self.gb1, self.gb2, self.gb3 = bf(P1, 16), bf(P2, 32), bf(P3, 64)
         self.loss = DeviceArray((1,), np.float64)
+        self.sloss = DeviceArray((1,), np.float64)
+        self.e1, self.e2, self.ge = f32(B, n, n, n), f32(B, n, n, n), f32(P1)
 
     def load_fields(self, src: DeviceArray, indices):
         """Copy fields src[i] (n^3 each) for i in indices into the batch."""
@@ -185,6 +187,33 @@ class UNet:
 
     def loss_value(self) -> float:
         return float(self.loss.numpy(self.s)[0]) / (self.B * (self.n - 2) ** 3)
+
+    def smooth_loss_value(self) -> float:
+        return float(self.sloss.numpy(self.s)[0]) / (self.B * (self.n - 2) ** 3)
+
+    def _smooth(self, x: DeviceArray, sweeps: int):
+        """x <- G0^sweeps x, G0 the JACOBI sweep with a zero shell (x's shell is
+        zero), on all B samples with the batched persistent kernel."""
+        nb = self.n ** 3 * 4
+        self.e2.copy_from(x, self.s)
+        probs = [dict(nx=self.n, ny=self.n, nz=self.n, radius=1, iterations=sweeps, op=OP_JACOBI,
+                      u1=x.ptr.value + b * nb, u2=self.e2.ptr.value + b * nb) for b in range(self.B)]
+        dispatch_batched(probs, [], BATCH_PERSISTENT, stream=self.s)
+
+    def add_smoothed_error_loss(self, target: DeviceArray, sweeps: int, weight: float):
+        """Adds weight * ||G0^K (pred - target)||^2 / count to the objective
+        (its gradient G0^K applied to 2 G0^K e, G0 being symmetric) on top of
+        the MSE gradient already in self.gu.  The error after K zero-boundary
+        JACOBI sweeps keeps the slowly decaying modes that set the number of
+        sweeps a warm start needs."""
+        P, s = self.P1, self.s
+        count = self.B * (self.n - 2) ** 3
+        nn.axpby(self.e1, 1.0, self.pred, -1.0, target, P, s)
+        self._smooth(self.e1, sweeps)
+        self.sloss.fill_zero(s)
+        nn.sumsq(self.e1, self.ge, self.sloss, P, weight / count, s)
+        self._smooth(self.ge, sweeps)
+        nn.axpby(self.gu, 1.0, self.gu, 1.0, self.ge, P, s)
 
     # -- backward
     def _layer_bwd(self, name, x_in, y_out, g_out, gb16, shape, P, g_in):
@@ -302,9 +331,12 @@ def relative_residual(u: DeviceArray, n: int, r0: float, stream=None) -> float:
 # ------------------------------------------------------------------ training
 
 def train(net: UNet, fields: DeviceArray, count: int, steps: int, lr: float = 1e-3,
-          lr_final: float = 1e-4, seed: int = 0, log_every: int = 50, verbose: bool = False):
+          lr_final: float = 1e-4, seed: int = 0, log_every: int = 50, verbose: bool = False,
+          smooth_sweeps: int = 0, smooth_weight: float = 0.0):
     """Adam on random batches of `fields` (the converged solves); the learning
-    rate decays geometrically from lr to lr_final.  Returns [(step, loss)]."""
+    rate decays geometrically from lr to lr_final.  With smooth_sweeps > 0 the
+    objective adds smooth_weight * ||G0^K (pred - target)||^2.  Returns
+    [(step, mse)] or [(step, mse, smoothed)]."""
     rng = np.random.default_rng(seed)
     order, pos, log = rng.permutation(count), 0, []
     t0 = time.perf_counter()
@@ -315,14 +347,17 @@ def train(net: UNet, fields: DeviceArray, count: int, steps: int, lr: float = 1e
         pos += net.B
         net.load_fields(fields, idx)
         net.forward(target=net.u)
+        if smooth_sweeps:
+            net.add_smoothed_error_loss(net.u, smooth_sweeps, smooth_weight)
         net.backward()
         rate = lr * (lr_final / lr) ** (step / max(1, steps - 1))
         net.adam_step(rate)
         if step % log_every == 0 or step == steps - 1:
             loss = net.loss_value()
-            log.append((step, loss))
+            rec = (step, loss) if not smooth_sweeps else (step, loss, net.smooth_loss_value())
+            log.append(rec)
             if verbose:
-                print(f"[train] step {step} loss {loss:.3e} lr {rate:.2e} "
+                print(f"[train] step {step} loss {rec[1:]} lr {rate:.2e} "
                       f"{time.perf_counter() - t0:.1f}s", flush=True)
     return log
 
@@ -403,3 +438,72 @@ def compare_warm_start(state: dict, seed: int, n: int = 128, m: int = 64, tol: f
     for t in (modes, u1, u2, feat, base):
         t.free()
     return out
+
+
+# ------------------------------------------------------------------ command line
+
+def main(argv=None):
+    """python -m thesis_amd.warmstart train|eval ...
+
+    train: generate `--count` converged `--n`^3 solves with the batched
+           Jacobi sweep, train the network, save the weights (.npz)
+    eval:  compare plain zero-start JACOBI with the learned warm start on
+           held-out 128^3 problems; prints one JSON line"""
+    import argparse
+
+    ap = argparse.ArgumentParser(prog="python -m thesis_amd.warmstart")
+    ap.add_argument("mode", choices=["train", "eval"])
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--count", type=int, default=4096)
+    ap.add_argument("--sweeps", type=int, default=12000)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=0x5EED0005)
+    ap.add_argument("--weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                      "tuned", "unet64.npz"))
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--fine", type=int, default=128)
+    ap.add_argument("--problems", type=int, default=3)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--check-every", type=int, default=200)
+    ap.add_argument("--smooth-sweeps", type=int, default=64)
+    ap.add_argument("--smooth-weight", type=float, default=100.0)
+    a = ap.parse_args(argv)
+    s = Stream()
+    if a.mode == "train":
+        t0 = time.perf_counter()
+        fields, _ = generate_solves(a.count, a.n, a.seed, a.sweeps, s)
+        t_gen = time.perf_counter() - t0
+        print(f"[warmstart] {a.count} solves of {a.n}^3 x {a.sweeps} sweeps: {t_gen:.1f} s", flush=True)
+        net = UNet(a.batch, a.n, stream=s)
+        t0 = time.perf_counter()
+        log = train(net, fields, a.count, a.steps, lr=a.lr, lr_final=a.lr / 10, log_every=100,
+                    verbose=True, smooth_sweeps=a.smooth_sweeps, smooth_weight=a.smooth_weight)
+        s.sync()
+        t_train = time.perf_counter() - t0
+        out = a.out or a.weights
+        net.save(out)
+        print(json.dumps({"generate_s": round(t_gen, 2), "train_s": round(t_train, 2),
+                          "ms_per_step": round(1e3 * t_train / a.steps, 2), "loss": log,
+                          "weights": out}), flush=True)
+    else:
+        with np.load(a.weights, allow_pickle=False) as z:
+            state = {k: z[k] for k in z.files}
+        results = []
+        for i in range(a.problems):
+            r = compare_warm_start(state, seed=a.seed + 1_000_000 + i, n=a.fine, m=a.n, tol=a.tol,
+                                   check_every=a.check_every, stream=s)
+            print(f"[warmstart] problem {i}: plain {r['plain']['sweeps']} sweeps "
+                  f"{r['plain']['seconds']:.3f} s, learned {r['learned']['sweeps']} sweeps "
+                  f"{r['learned']['seconds']:.3f} s (ratio {r['time_ratio_learned_vs_plain']:.3f}), "
+                  f"blend {r['blend']['sweeps']} sweeps", flush=True)
+            results.append(r)
+        ratios = [r["time_ratio_learned_vs_plain"] for r in results]
+        print(json.dumps({"metric": "time-to-1e-6 residual, learned warm start vs plain zero start",
+                          "value": float(np.mean(ratios)), "unit": "ratio", "higher_is_better": False,
+                          "results": results}), flush=True)
+
+
+if __name__ == "__main__":
+    main()
