@@ -170,12 +170,13 @@ def probe(st, np):
     st.copy_f32(d2, d1, N ** 3)
     st.synchronize()
     print(f"[probe] copy {N}^3 fp32 ({4 * N ** 3} bytes read, {4 * N ** 3} written)", flush=True)
-    for v in ("smem_register", "base", "smem_padded", "coop"):
-        q = p.copy(variant=v)
+    for v, its, form in (("smem_register", 2, 0), ("base", 2, 0), ("smem_padded", 2, 0),
+                         ("coop", 2, 1), ("coop", SWEEPS, 3)):
+        q = p.copy(variant=v, iterations=its, temporal_form=form)
         st.initialize_region(q, d1, d2, st.INIT_SEEDED, seed=SEED)
         st.dispatch_kernels(q, d1, d2)
         st.synchronize()
-        print(f"[probe] {v}: 2 sweeps, launch {st.launch_config(q)}", flush=True)
+        print(f"[probe] {v} form {form}: {its} sweeps, launch {st.launch_config(q)}", flush=True)
 
 
 def main():
@@ -197,7 +198,12 @@ def main():
                          "against the FP64 oracle")
     ap.add_argument("--probe", action="store_true",
                     help="short run for counter collection: one streaming copy of a 512^3 "
-                         "field, then 2 sweeps of each kernel family")
+                         "field, 2 sweeps of each single-sweep family and of the LDS-ring "
+                         "COOP form, then one 100-sweep launch of the streaming COOP form")
+    ap.add_argument("--init", default="seeded", choices=["seeded", "reference"],
+                    help="seeded: the same boundary in both buffers (two sweeps fuse per pass "
+                         "in the streaming form); reference: u2 zero (different shells, one "
+                         "sweep per pass)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -240,7 +246,8 @@ def main():
     # The field is initialised once; every step advances it by 100 more
     # sweeps (an even count, so the state stays in d1).  The work per sweep
     # does not depend on the values.
-    st.initialize_region(p, d1, d2, st.INIT_SEEDED, seed=SEED, stream=stream)
+    st.initialize_region(p, d1, d2, st.INIT_SEEDED if args.init == "seeded" else st.INIT_REFERENCE,
+                         seed=SEED, stream=stream)
 
     def step():
         in_u2, ms = st.dispatch_kernels(p, d1, d2, stream=stream, timed=True)
@@ -279,7 +286,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: seeded uniform [0,1) boundary shell, zero interior",
+        "data": ("synthetic: seeded uniform [0,1) boundary shell, zero interior" if args.init == "seeded"
+                 else "synthetic: reference initial condition (u1 shell 1, u2 zero)"),
         "config": {"workload": "3D 7-point JACOBI u'=(1/6)*sum of 6 neighbours, 512^3 FP32, "
                                "radius 1, 100 sweeps per step",
                    "variant": variant_name, "launch": cfg, "device": st.device_name(0)},

@@ -103,7 +103,10 @@ typedef struct st_params {
     int block_y;          /*   SMEM_REGISTER: block_x is a multiple of 64 (one */
     int block_z;          /*   wave across x), block_z is the z-chunk depth in */
                           /*   planes (0 = whole axis split heuristically).    */
-                          /*   COOP: block_z is ignored.                       */
+                          /*   COOP: block_z is ignored, except by the         */
+                          /*   streaming form (temporal_form 3), where block_y */
+                          /*   is the waves per frame (2 rows each) and        */
+                          /*   block_z the z-chunk depth.                      */
     int coarsen_x;        /* COARSEN_X: cells per thread along x (>= 1)        */
     int coarsen_y;        /* SMEM_REGISTER/COOP: rows per thread (0=heuristic) */
     int smem_pad;         /* SMEM_PAD: extra LDS floats per tile row           */
@@ -120,7 +123,11 @@ typedef struct st_params {
     int temporal_form;    /* COOP: 0 = library's choice, 1 = LDS-ring tiles
                              (F32/F64, radius 1..4, any pass length),
                              2 = register-queue full-row frames (F32, radius
-                             1, temporal_steps 2, even iteration count)        */
+                             1, temporal_steps 2, even iteration count),
+                             3 = streaming rows, one wave per grid row (F32,
+                             radius 1, nx a multiple of 4 up to 512,
+                             cooperative launch; two fused sweeps
+                             when both buffers hold the same shell, else one) */
     double weight;        /* JACOBI neighbour weight (0 = 1/(2*D*R)); rounded
                              to the storage type                               */
 } st_params;

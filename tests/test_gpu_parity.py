@@ -306,3 +306,81 @@ def test_coop_register_form_rejects_unsupported(gpu):
                                   heuristic=1, op=1, **kw)
         with pytest.raises(st.StencilError):
             st.check_early_exit(p)
+
+
+# streaming temporal form (whole rows per wave; two fused sweeps when both
+# buffers hold the same shell, one sweep per pass otherwise)
+STREAM_SHAPES = [
+    ((512, 60, 40), {}),                      # 8 cells per lane, several frames in y
+    ((64, 40, 37), {}),                       # 4 cells per lane
+    ((264, 50, 30), {}),                      # 8 cells per lane, idle quads past the row
+    ((260, 41, 12), {"block_y": 5, "heuristic": 0}),   # second quad holds one column pair
+    ((8, 9, 5), {}),                          # one lane per row
+    ((256, 300, 9), {}),                      # many frames, thin z
+    ((16, 3, 3), {}),                         # a single interior cell
+    ((128, 70, 61), {"block_z": 17, "heuristic": 0}),  # explicit z chunk
+    ((512, 60, 40), {"coarsen_y": 2, "block_y": 12, "heuristic": 0}),  # 2 rows of 8 cells
+    ((200, 45, 21), {"coarsen_y": 4, "block_y": 4, "heuristic": 0}),   # 4 rows of 4 cells
+]
+
+
+@pytest.mark.parametrize("shape,knobs", STREAM_SHAPES, ids=str)
+@pytest.mark.parametrize("op", [0, 1])
+@pytest.mark.parametrize("init", ["seeded", "reference"])
+def test_coop_stream_form(gpu, oracle, shape, knobs, op, init):
+    st = gpu
+    nx, ny, nz = shape
+    if init == "reference":
+        u1 = oracle.init_reference(nx, ny, nz, 3)
+        u2 = np.zeros_like(u1)
+    else:
+        u1 = oracle.init_seeded(nx, ny, nz, 3, 1, 0x5EED0006)
+        rng = np.random.default_rng(nx + ny + nz)
+        u1 = np.where(u1 == 0, rng.random(u1.shape), u1).astype(np.float32)
+        u2 = u1.copy()
+    kw = dict(heuristic=1)
+    kw.update(knobs)
+    if kw["heuristic"] == 0:
+        kw.setdefault("block_y", 10)  # waves per frame
+        kw["block_x"] = nx
+    for its in (1, 2, 3, 4, 5, 6, 7, 12):
+        _announce("coop stream form", shape, knobs, op, init, its)
+        want = oracle.run(u1.copy(), u2.copy(), 3, 1, op, its).copy()
+        p = st.StencilParams.make(nx=nx, ny=ny, nz=nz, dimensions=3, radius=1, iterations=its,
+                                  op=op, variant="coop", temporal_form=3, **kw)
+        cfg = st.launch_config(p)
+        assert cfg["launches"] == 1
+        got, in_u2 = run_gpu(st, p, u1, u2)
+        assert in_u2 == bool(its % 2)
+        assert_bitwise(got, want, f"stream form {shape} {knobs} op={op} {init} its={its}")
+
+
+@pytest.mark.parametrize("face", ["x0", "x1", "y0", "y1", "z0", "z1", "edge", "corner"])
+def test_coop_stream_one_shell_cell_differs(gpu, oracle, face):
+    """Buffers whose shells differ in one cell must take the per-sweep path
+    (the fused path would use u1's value for u2's shell)."""
+    st = gpu
+    nx, ny, nz = 64, 33, 29
+    u1 = oracle.init_seeded(nx, ny, nz, 3, 1, 0x5EED0007)
+    u2 = u1.copy()
+    cell = {"x0": (14, 9, 0), "x1": (14, 9, nx - 1), "y0": (7, 0, 30), "y1": (7, ny - 1, 30),
+            "z0": (0, 16, 40), "z1": (nz - 1, 16, 40), "edge": (5, 0, nx - 1),
+            "corner": (nz - 1, ny - 1, 0)}[face]
+    u2[cell] = np.float32(0.75)
+    _announce("stream shell cell", face, cell)
+    for its in (4, 6):
+        want = oracle.run(u1.copy(), u2.copy(), 3, 1, 1, its).copy()
+        p = st.StencilParams.make(nx=nx, ny=ny, nz=nz, dimensions=3, radius=1, iterations=its,
+                                  op=1, variant="coop", temporal_form=3, heuristic=1)
+        got, _ = run_gpu(st, p, u1, u2)
+        assert_bitwise(got, want, f"shell cell {face} its={its}")
+
+
+def test_coop_stream_form_rejects_unsupported(gpu):
+    st = gpu
+    for kw in (dict(nx=62, ny=40, nz=40), dict(nx=520, ny=40, nz=40), dict(dim=40, radius=2),
+               dict(dim=40, dtype=st.F64), dict(dim=40, launch_per_pass=1)):
+        p = st.StencilParams.make(variant="coop", temporal_form=3, heuristic=1, op=1,
+                                  iterations=4, **kw)
+        with pytest.raises(st.StencilError):
+            st.check_early_exit(p)

@@ -103,8 +103,9 @@ static st_status validate_common(const st_params *p, bool mixed)
     }
     if (p->coarsen_x < 0 || p->coarsen_y < 0 || p->smem_pad < 0 || p->temporal_steps < 0 ||
         p->launch_per_pass < 0 || p->launch_per_pass > 1 || p->temporal_form < 0 ||
-        p->temporal_form > 2)
-        return fail(ST_EINVAL, "negative coarsening, padding or temporal steps");
+        p->temporal_form > 3)
+        return fail(ST_EINVAL, "coarsening, padding, temporal steps, launch_per_pass or "
+                               "temporal_form out of range");
     const bool heur = p->heuristic || p->block_x <= 0;
     if (!heur) {
         if (p->block_x < 1 || p->block_y < 1 || (p->dimensions == 3 && p->block_z < 1))
@@ -159,13 +160,20 @@ static st_status resolve(const st_params *p, Cfg &c, bool mixed)
     c.gx = c.gy = c.gz = 1;
     if (mixed) return ST_OK;
     if (c.variant == ST_V_AUTO) {
-        // Register march wherever it is instantiated, the cooperative
-        // temporal-blocked kernel is opted into explicitly.
-        const bool march_ok = (c.dtype == ST_F32 && c.r <= 8) ||
-                              (c.dtype == ST_F64 && (c.r == 1 || c.r == 2 || c.r == 4));
-        c.variant = march_ok ? ST_V_SMEM_REGISTER : ST_V_BASE;
+        // The streaming temporal form (two sweeps per HBM pass) where it
+        // applies and at least one fused pass runs; else the register march
+        // wherever it is instantiated; else the plain kernel.
         c.bx = c.by = c.bz = 0;
         c.heuristic = 1;
+        if (c.iters >= 4 && (c.tsteps == 0 || c.tsteps == 2) && c.form == 0 && stream_eligible(c)) {
+            c.variant = ST_V_COOP;
+            c.cx = 1;
+            c.cy = 0;
+        } else {
+            const bool march_ok = (c.dtype == ST_F32 && c.r <= 8) ||
+                                  (c.dtype == ST_F64 && (c.r == 1 || c.r == 2 || c.r == 4));
+            c.variant = march_ok ? ST_V_SMEM_REGISTER : ST_V_BASE;
+        }
     }
     switch (c.variant) {
     case ST_V_BASE: return resolve_base(c);
@@ -236,6 +244,7 @@ st_status stencil_launch_config(const st_params *p, st_launch_info *info)
             int n = c.iters > 0 ? (c.iters + c.tsteps - 1) / c.tsteps : 0;
             if (n && (n & 1) != (c.iters & 1)) ++n;
             if (c.kind == 2) n = c.iters / c.tsteps;
+            if (c.kind == 3) n = c.iters > 0 ? 1 : 0;
             info->launches = n ? (c.per_pass ? n : 1) : 0;
         } else {
             info->launches = c.iters;

@@ -58,6 +58,11 @@ bool tblock_eligible(const Cfg &c, int t);
 st_status resolve_tblock(Cfg &c, int t);
 st_status run_tblock(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2);
 
+// Streaming temporal blocking, whole rows per wave (sweep_stream.hip).
+bool stream_eligible(const Cfg &c);
+st_status resolve_stream(Cfg &c);
+st_status run_stream(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2);
+
 st_status launch_init(const Cfg &c, int kind, unsigned long long seed, void *u1, void *u2,
                       hipStream_t s);
 st_status residual_norm(const Cfg &c, const void *u, double *norm, hipStream_t s);

@@ -209,6 +209,13 @@ st_status resolve_coop(Cfg &c)
     if (c.dims != 3) return fail(ST_EUNSUPPORTED, "COOP is implemented for 3D grids");
     if (c.r > 4) return fail(ST_EUNSUPPORTED, "COOP is built for radius 1..4");
     const bool heur = c.heuristic || c.bx <= 0;
+    // streaming form: requested, or the library's choice for heuristic
+    // shapes where it applies (two fused sweeps)
+    if (c.form == 3 || (c.form == 0 && heur && (c.tsteps == 0 || c.tsteps == 2) && stream_eligible(c))) {
+        c.zchunk = (c.heuristic || c.bz <= 0) ? 0 : c.bz;
+        if (c.heuristic) c.by = 0;
+        return resolve_stream(c);
+    }
     {
         // register-queue form: requested, or the library's choice for
         // heuristic shapes where it applies
@@ -278,6 +285,7 @@ st_status resolve_coop(Cfg &c)
 st_status run_coop(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2)
 {
     if (c.kind == 2) return run_tblock(c, u1, u2, s, in_u2);
+    if (c.kind == 3) return run_stream(c, u1, u2, s, in_u2);
     const void *fn = pick_coop(c);
     if (!fn) return fail(ST_EUNSUPPORTED, "no COOP kernel for radius %d", c.r);
     CoopArgs a;

@@ -1,0 +1,547 @@
+// COOP family, streaming form: two fused sweeps per HBM pass, whole grid
+// rows per wave.
+//
+// Contract (as the other COOP forms): `iters` sweeps of the per-sweep
+// ping-pong (sweep s reads buffer s%2, writes the interior of the other),
+// result in u2 exactly when iters is odd.  Both buffers' outermost shells
+// are compared on the GPU first (k_shell_eq):
+//
+//   * equal shells (ST_INIT_SEEDED, or any state where both buffers hold the
+//     same boundary): a shell cell has the same value at every time level,
+//     so two sweeps fuse into one pass with no boundary reloads.  Passes of
+//     two sweeps run first, then (iters mod 4) single-sweep passes, which
+//     keeps the pass count's parity equal to the iteration count's;
+//   * different shells (ST_INIT_REFERENCE leaves u2 zero): one sweep per
+//     pass, reading the whole of buffer s%2, as the reference does.
+//
+// Layout: a workgroup of WY waves owns a frame of 2*WY full rows (a wave
+// holds two rows, each lane CX cells of both, in quads of four consecutive
+// columns 256 apart) and marches it
+// along z over a chunk of planes.  Per level and plane:
+//   x neighbours   own registers, lane edges by DPP wave shifts (the row
+//                  fits one wave, so the only missing neighbours are the
+//                  ones outside the grid, next to shell cells);
+//   y neighbours   own registers, frame-row edges through one LDS exchange
+//                  of both rows (one barrier);
+//   z neighbours   a running partial sum: when plane k arrives its x, y
+//                  and k-1 terms are added (in the reference order) and the
+//                  k+1 term completes it one plane later.
+// Level 0 of the next plane is prefetched into registers before the current
+// plane is processed; global loads stay in flight across the LDS barriers.
+// Frames overlap by two rows on each side (one per fused sweep) and only
+// their core rows are stored; z chunks overlap by two planes likewise.
+#include <hip/hip_cooperative_groups.h>
+
+#include "st_device.hpp"
+#include "st_internal.hpp"
+
+namespace st {
+
+constexpr int kShellBlocks = 256;
+
+struct SmArgs {
+    int nx, ny, nz;
+    long long sz;         // nx*ny
+    int core_h;           // stored rows per frame (frame rows - 4)
+    int tiles_y, zchunk, ntiles;
+    int iters;
+    int n2, n1;           // equal shells: passes of two sweeps, then of one
+    const int *shell_eq;  // kShellBlocks flags from k_shell_eq
+};
+
+// part[b] = 1 when every shell cell block b visits holds the same bits in
+// u1 and u2.
+__global__ __launch_bounds__(256) void k_shell_eq(const float *__restrict__ u1,
+                                                  const float *__restrict__ u2, int nx, int ny,
+                                                  int nz, int *__restrict__ part)
+{
+    const long long A = (long long)nx * ny;
+    const long long P = 2LL * nx + 2LL * (ny - 2);  // shell cells of an inner plane
+    const long long n = 2 * A + (long long)(nz - 2) * P;
+    int ok = 1;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) {
+        long long idx;
+        if (i < 2 * A) {
+            const long long j = i < A ? i : i - A;
+            idx = (i < A ? 0 : (long long)(nz - 1) * A) + j;
+        } else {
+            const long long q = i - 2 * A;
+            const long long z = 1 + q / P;
+            long long j = q % P;
+            int x, y;
+            if (j < 2LL * nx) {
+                y = j < nx ? 0 : ny - 1;
+                x = (int)(j < nx ? j : j - nx);
+            } else {
+                j -= 2LL * nx;
+                y = 1 + (int)(j >> 1);
+                x = (j & 1) ? nx - 1 : 0;
+            }
+            idx = z * A + (long long)y * nx + x;
+        }
+        ok &= __float_as_uint(u1[idx]) == __float_as_uint(u2[idx]);
+    }
+    ok = __syncthreads_and(ok);
+    if (threadIdx.x == 0) part[blockIdx.x] = ok;
+}
+
+// Lane i receives v of lane i-1 / i+1, rotating at the wave ends (wave-wide
+// DPP rotates).
+__device__ __forceinline__ float from_prev_lane(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x13C, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_next_lane(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134, 0xF, 0xF, false));
+}
+
+// A lane's cells of one row: CX = 4*H values, quad h holding columns
+// 256*h + 4*lane + 0..3, so that one 16-byte access per lane covers 1 KiB of
+// the row contiguously (whole cache lines on every load and store).
+template <int CX, int RPW>
+struct SmPlane {
+    float v[RPW][CX];
+};
+
+// Per-lane column constants.
+template <int CX>
+struct SmLane {
+    int xo[CX / 4];    // first column of quad h, clamped into the row
+    bool act[CX / 4];  // quad h lies inside the row
+    bool lo[CX / 4];   // quad h starts at column 0
+    bool hi[CX / 4];   // quad h ends at column nx-1
+};
+
+// Per-tile, per-wave constants.
+template <int RPW>
+struct SmTile {
+    int rowoff[RPW];            // clamped row offsets within a plane
+    bool yshell[RPW];           // row outside the interior (wave-uniform)
+    bool ystore[RPW];           // row is an interior core row (wave-uniform)
+    int z0, z1;
+};
+
+template <int CX, int RPW>
+__device__ __forceinline__ void sm_load(SmPlane<CX, RPW> &d, const float *__restrict__ X, const SmArgs &a,
+                                        const SmTile<RPW> &t, const SmLane<CX> &l, int k)
+{
+    typedef float V4 __attribute__((ext_vector_type(4)));
+    k = min(max(k, 0), a.nz - 1);
+    const float *base = X + (long long)k * a.sz;
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+#pragma unroll
+        for (int h = 0; h < CX / 4; ++h) {
+            const V4 q = *reinterpret_cast<const V4 *>(base + t.rowoff[r] + l.xo[h]);
+            d.v[r][4 * h + 0] = q[0];
+            d.v[r][4 * h + 1] = q[1];
+            d.v[r][4 * h + 2] = q[2];
+            d.v[r][4 * h + 3] = q[3];
+        }
+    }
+}
+
+// Partial sums of the plane `nw` (x, y, then the z-previous plane `zm`) in
+// the reference's order; the y edges come from the neighbouring waves
+// through `reg` (one LDS region of 2*WY*CX/4*64 float4).
+template <int CX, int RPW>
+__device__ __forceinline__ void sm_partial(SmPlane<CX, RPW> &out, const SmPlane<CX, RPW> &nw,
+                                           const SmPlane<CX, RPW> &zm, float4 *reg, int w, int wy,
+                                           int lane)
+{
+    constexpr int H = CX / 4;
+    // edge rows only: row 0 (e = 0) and row RPW-1 (e = 1)
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int r = e ? RPW - 1 : 0;
+#pragma unroll
+        for (int h = 0; h < H; ++h)
+            reg[((w * 2 + e) * H + h) * 64 + lane] =
+                make_float4(nw.v[r][4 * h], nw.v[r][4 * h + 1], nw.v[r][4 * h + 2], nw.v[r][4 * h + 3]);
+    }
+    __syncthreads();
+    const int wm = w == 0 ? wy - 1 : w - 1;   // wrapped rows are outside the
+    const int wp = w == wy - 1 ? 0 : w + 1;   // frame core and never stored
+    float ym[CX], yp[CX];
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+        const float4 m = reg[((wm * 2 + 1) * H + h) * 64 + lane];
+        const float4 p = reg[((wp * 2 + 0) * H + h) * 64 + lane];
+        ym[4 * h] = m.x; ym[4 * h + 1] = m.y; ym[4 * h + 2] = m.z; ym[4 * h + 3] = m.w;
+        yp[4 * h] = p.x; yp[4 * h + 1] = p.y; yp[4 * h + 2] = p.z; yp[4 * h + 3] = p.w;
+    }
+    // y neighbours per row as values (no select between arrays, which would
+    // keep them in scratch memory)
+    float ya[RPW][CX], yb[RPW][CX];
+#pragma unroll
+    for (int c = 0; c < CX; ++c) {
+        ya[0][c] = ym[c];
+        yb[RPW - 1][c] = yp[c];
+    }
+#pragma unroll
+    for (int r = 1; r < RPW; ++r)
+#pragma unroll
+        for (int c = 0; c < CX; ++c) {
+            ya[r][c] = nw.v[r - 1][c];
+            yb[r - 1][c] = nw.v[r][c];
+        }
+    const bool first = lane == 0, last = lane == 63;
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        // column 4*lane-1 of quad h is lane-1's last cell of quad h, except
+        // on lane 0 where it is lane 63's last cell of quad h-1; likewise on
+        // the right
+        float pa[H], pb[H];
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+            pa[h] = from_prev_lane(nw.v[r][4 * h + 3]);
+            pb[h] = from_next_lane(nw.v[r][4 * h]);
+        }
+        float xa[CX], xb[CX];
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+            const float lft = h > 0 ? pa[h - 1] : pa[0];
+            const float rgt = h < H - 1 ? pb[h + 1] : pb[H - 1];
+            xa[4 * h] = first ? lft : pa[h];
+            xb[4 * h + 3] = last ? rgt : pb[h];
+#pragma unroll
+            for (int j = 1; j < 4; ++j) {
+                xa[4 * h + j] = nw.v[r][4 * h + j - 1];
+                xb[4 * h + j - 1] = nw.v[r][4 * h + j];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CX; ++c) {
+            float s = 0.0f;
+            s += xa[c];
+            s += xb[c];
+            s += ya[r][c];
+            s += yb[r][c];
+            s += zm.v[r][c];
+            out.v[r][c] = s;
+        }
+    }
+}
+
+// out = finish(partial + z-next) with cells outside the interior taking the
+// centre value (equal shells: a shell cell never changes).
+template <int OP, int CX, int RPW>
+__device__ __forceinline__ void sm_finish(SmPlane<CX, RPW> &out, const SmPlane<CX, RPW> &part,
+                                          const SmPlane<CX, RPW> &zp, const SmPlane<CX, RPW> &cen,
+                                          float kc, bool zshell, const SmTile<RPW> &t,
+                                          const SmLane<CX> &l)
+{
+#pragma unroll
+    for (int r = 0; r < RPW; ++r)
+#pragma unroll
+        for (int c = 0; c < CX; ++c) out.v[r][c] = finish<OP>(part.v[r][c] + zp.v[r][c], cen.v[r][c], kc);
+    if (zshell) {
+        out = cen;
+        return;
+    }
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        if (t.yshell[r]) {
+#pragma unroll
+            for (int c = 0; c < CX; ++c) out.v[r][c] = cen.v[r][c];
+        } else {
+#pragma unroll
+            for (int h = 0; h < CX / 4; ++h) {
+                const float a0 = cen.v[r][4 * h], b0 = out.v[r][4 * h];
+                const float a1 = cen.v[r][4 * h + 3], b1 = out.v[r][4 * h + 3];
+                out.v[r][4 * h] = l.lo[h] ? a0 : b0;
+                out.v[r][4 * h + 3] = l.hi[h] ? a1 : b1;
+            }
+        }
+    }
+}
+
+template <int CX, int RPW>
+__device__ __forceinline__ void sm_store(float *__restrict__ Y, const SmPlane<CX, RPW> &v,
+                                         const SmArgs &a, const SmTile<RPW> &t, const SmLane<CX> &l,
+                                         int k)
+{
+    if (k < t.z0 || k >= t.z1 || k < 1 || k > a.nz - 2) return;
+    typedef float V4 __attribute__((ext_vector_type(4)));
+    float *base = Y + (long long)k * a.sz;
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        if (!t.ystore[r]) continue;
+        float *row = base + t.rowoff[r];
+#pragma unroll
+        for (int h = 0; h < CX / 4; ++h) {
+            if (l.act[h] && !l.lo[h] && !l.hi[h]) {
+                V4 q = {v.v[r][4 * h], v.v[r][4 * h + 1], v.v[r][4 * h + 2], v.v[r][4 * h + 3]};
+                __builtin_nontemporal_store(q, reinterpret_cast<V4 *>(row + l.xo[h]));
+            } else if (l.act[h]) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (!(j == 0 && l.lo[h]) && !(j == 3 && l.hi[h])) row[l.xo[h] + j] = v.v[r][4 * h + j];
+            }
+        }
+    }
+}
+
+// One pass of T (1 or 2) sweeps X -> Y over all tiles of this workgroup.
+template <int OP, int CX, int RPW, int T>
+__device__ __forceinline__ void sm_pass(const float *__restrict__ X, float *__restrict__ Y,
+                                        const SmArgs &a, float kc, float4 *lds, unsigned bid,
+                                        unsigned nblk)
+{
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wy = blockDim.x >> 6;
+    SmLane<CX> l;
+#pragma unroll
+    for (int h = 0; h < CX / 4; ++h) {
+        const int x = 256 * h + 4 * lane;
+        l.act[h] = x < a.nx;
+        l.xo[h] = l.act[h] ? x : a.nx - 4;  // quads past the row reload its last cells
+        l.lo[h] = x == 0;
+        l.hi[h] = x + 4 == a.nx;
+    }
+    float4 *reg0 = lds;
+    float4 *reg1 = lds + 2 * wy * (CX / 4) * 64;
+
+    for (int tile = (int)bid; tile < a.ntiles; tile += (int)nblk) {
+        const int tyi = tile % a.tiles_y, tzi = tile / a.tiles_y;
+        SmTile<RPW> t;
+        const int fy0 = tyi * a.core_h - 2;
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int fr = w * RPW + r;  // frame row
+            const int gy = fy0 + fr;
+            const int gyc = min(max(gy, 0), a.ny - 1);
+            t.rowoff[r] = gyc * a.nx;
+            t.yshell[r] = gy <= 0 || gy >= a.ny - 1;
+            t.ystore[r] = !t.yshell[r] && fr >= 2 && fr < 2 + a.core_h;
+        }
+        t.z0 = tzi * a.zchunk;
+        t.z1 = min(t.z0 + a.zchunk, a.nz);
+
+        // Pipeline state: the prefetched level-0 plane, the previous level-0
+        // and level-1 planes, and the running partial sums P1/P2 (updated in
+        // place once the finished plane has taken their old values).
+        SmPlane<CX, RPW> pf, c0, c1, P1, P2;
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int c = 0; c < CX; ++c) c0.v[r][c] = c1.v[r][c] = P1.v[r][c] = P2.v[r][c] = 0.0f;
+        const int kin_lo = t.z0 - T, kin_hi = t.z1 + T;
+        sm_load<CX, RPW>(pf, X, a, t, l, kin_lo);
+        for (int kin = kin_lo; kin < kin_hi; ++kin) {
+            const SmPlane<CX, RPW> nw0 = pf;
+            if (kin + 1 < kin_hi) sm_load<CX, RPW>(pf, X, a, t, l, kin + 1);
+            // level 1, plane kin-1
+            const int k1 = kin - 1;
+            SmPlane<CX, RPW> o1;
+            sm_finish<OP, CX, RPW>(o1, P1, nw0, c0, kc, k1 <= 0 || k1 >= a.nz - 1, t, l);
+            // one sweep: alternate the two regions so that a plane's LDS
+            // writes never meet the previous plane's reads
+            sm_partial<CX, RPW>(P1, nw0, c0, (T == 1 && (kin & 1)) ? reg1 : reg0, w, wy, lane);
+            c0 = nw0;
+            if constexpr (T == 1) {
+                sm_store<CX, RPW>(Y, o1, a, t, l, k1);
+            } else {
+                // level 2, plane kin-2
+                const int k2 = kin - 2;
+                SmPlane<CX, RPW> o2;
+                sm_finish<OP, CX, RPW>(o2, P2, o1, c1, kc, k2 <= 0 || k2 >= a.nz - 1, t, l);
+                sm_partial<CX, RPW>(P2, o1, c1, reg1, w, wy, lane);
+                c1 = o1;
+                sm_store<CX, RPW>(Y, o2, a, t, l, k2);
+            }
+        }
+        __syncthreads();  // the next tile rewrites both LDS regions
+    }
+}
+
+// Threads per workgroup from the state per lane (five planes of CX*RPW
+// values plus the plane in flight): 8 values fit the 128 VGPRs of 16 waves,
+// 16 the 168 of 12 waves, 32 the 256 of 8 waves.
+template <int CX, int RPW>
+constexpr int sm_max_threads() { return CX * RPW <= 8 ? 1024 : CX * RPW <= 16 ? 768 : 512; }
+
+template <int OP, int CX, int RPW>
+__global__ __launch_bounds__((sm_max_threads<CX, RPW>())) void k_stream(float *u1, float *u2, SmArgs a,
+                                                                      float kc)
+{
+    extern __shared__ float4 lds[];
+    int ok = 1;
+    for (int i = threadIdx.x; i < kShellBlocks; i += blockDim.x) ok &= a.shell_eq[i];
+    const bool same = __syncthreads_and(ok) != 0;
+    const unsigned nblk = gridDim.x;
+    const unsigned bid = xcd_remap(blockIdx.x, nblk);
+    const int npass = same ? a.n2 + a.n1 : a.iters;
+    for (int p = 0; p < npass; ++p) {
+        const float *X = (p & 1) ? u2 : u1;
+        float *Y = (p & 1) ? u1 : u2;
+        if (same && p < a.n2) sm_pass<OP, CX, RPW, 2>(X, Y, a, kc, lds, bid, nblk);
+        else sm_pass<OP, CX, RPW, 1>(X, Y, a, kc, lds, bid, nblk);
+        if (p + 1 < npass) cooperative_groups::this_grid().sync();
+    }
+}
+
+// ---------------------------------------------------------------- host side
+
+static int stream_cx(const Cfg &c)
+{
+    if (c.nx % 4 || c.nx > 512) return 0;
+    return c.nx <= 256 ? 4 : 8;
+}
+
+bool stream_eligible(const Cfg &c)
+{
+    return c.dtype == ST_F32 && c.dims == 3 && c.r == 1 && c.iters >= 1 && !c.per_pass &&
+           c.nx >= 8 && c.ny >= 3 && c.nz >= 3 && stream_cx(c) != 0;
+}
+
+template <int CX, int RPW>
+static const void *stream_fn_t(int op)
+{
+    return op == OP_THESIS ? reinterpret_cast<const void *>(&k_stream<OP_THESIS, CX, RPW>)
+                           : reinterpret_cast<const void *>(&k_stream<OP_JACOBI, CX, RPW>);
+}
+
+// Instantiated shapes: 8 cells x 2 rows (nx up to 512), 4 cells x 2 or 4
+// rows (nx up to 256).  8 x 4 needs more than 256 VGPRs and is not built.
+static const void *stream_fn(int op, int cx, int rpw)
+{
+    if (cx == 8) return stream_fn_t<8, 2>(op);
+    return rpw == 4 ? stream_fn_t<4, 4>(op) : stream_fn_t<4, 2>(op);
+}
+
+static int stream_max_waves(int cx, int rpw)
+{
+    if (cx == 8) return sm_max_threads<8, 2>() / 64;
+    return (rpw == 4 ? sm_max_threads<4, 4>() : sm_max_threads<4, 2>()) / 64;
+}
+
+// Tiles of one pass for `wy` waves of `rpw` rows per frame: y frames times
+// z chunks, the chunk count filling the CUs once.
+struct SmPlan {
+    int wy, core_h, tiles_y, chunk, ntiles;
+    double cost;
+};
+
+static SmPlan stream_plan(const Cfg &c, int wy, int rpw, int cus, int chunk_req)
+{
+    SmPlan p;
+    p.wy = wy;
+    p.core_h = wy * rpw - 4;
+    p.tiles_y = (c.ny + p.core_h - 1) / p.core_h;
+    int chunk = chunk_req;
+    if (chunk <= 0) {
+        int chunks = cus / p.tiles_y;
+        if (chunks < 1) chunks = 1;
+        chunk = (c.nz + chunks - 1) / chunks;
+        if (chunk < 4) chunk = 4;
+        if (chunk > c.nz) chunk = c.nz;
+    }
+    p.chunk = chunk;
+    p.ntiles = p.tiles_y * ((c.nz + chunk - 1) / chunk);
+    // time of the busiest CU: tile rounds x planes marched x HBM rows per
+    // plane (core rows read and written; the four halo rows mostly hit L2)
+    const int rounds = (p.ntiles + cus - 1) / cus;
+    p.cost = (double)rounds * (chunk + 4) * (2.0 * p.core_h + 1.2);
+    return p;
+}
+
+st_status resolve_stream(Cfg &c)
+{
+    if (!stream_eligible(c))
+        return fail(ST_EUNSUPPORTED, "streaming temporal form needs FP32, 3D, radius 1, nx a multiple "
+                                     "of 4 up to 512, one cooperative launch");
+    const int cx = stream_cx(c);
+    // rows per wave: 2, or coarsen_y = 4 with 4 cells per lane (nx <= 256)
+    int rpw = 2;
+    if (!c.heuristic && c.cy > 0) {
+        if (c.cy != 2 && !(c.cy == 4 && cx == 4))
+            return fail(ST_EINVAL, "streaming form: coarsen_y must be 2, or 4 when nx <= 256");
+        rpw = c.cy;
+    }
+    const int wmax = stream_max_waves(cx, rpw);
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 256;
+    const int chunk_req = c.zchunk > 0 ? c.zchunk : 0;
+    const int wmin = (4 + rpw) / rpw;  // at least one core row
+    SmPlan best;
+    if (!c.heuristic && c.by > 0) {
+        // block_y = waves per frame (frame of rpw*block_y rows)
+        if (c.by < wmin || c.by > wmax)
+            return fail(ST_EINVAL, "streaming form: block_y (waves per frame) must be %d..%d", wmin, wmax);
+        best = stream_plan(c, c.by, rpw, cus, chunk_req);
+    } else {
+        // whole grid in one frame if it fits, else the frame height whose
+        // tiles fill the CUs best
+        const int wfit = (c.ny + 4 + rpw - 1) / rpw;
+        best = stream_plan(c, wfit < wmax ? (wfit < wmin ? wmin : wfit) : wmax, rpw, cus, chunk_req);
+        if (wfit > wmax)
+            for (int wy = wmax - 1; wy >= (wmax * 2) / 3 && wy >= wmin; --wy) {
+                const SmPlan q = stream_plan(c, wy, rpw, cus, chunk_req);
+                if (q.cost < best.cost * 0.97) best = q;  // fewer waves must pay for themselves
+            }
+    }
+    c.kind = 3;
+    c.cx = cx;
+    c.cy = rpw;
+    c.wx = 1;
+    c.wy = best.wy;
+    c.tsteps = 2;
+    c.lds = 2 * (2 * best.wy * (cx / 4) * 64 * 16);
+    c.bx = c.nx;
+    c.by = best.core_h;
+    c.zchunk = best.chunk;
+    c.bz = best.chunk;
+    c.gx = 0;
+    c.gy = c.gz = 1;
+    return ST_OK;
+}
+
+st_status run_stream(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2)
+{
+    *in_u2 = c.iters & 1;
+    if (c.iters <= 0) return ST_OK;
+    void *ws = nullptr;
+    st_status st = workspace_slot(3, kShellBlocks * sizeof(int), &ws);
+    if (st != ST_OK) return st;
+    hipLaunchKernelGGL(k_shell_eq, dim3(kShellBlocks), dim3(256), 0, s, static_cast<const float *>(u1),
+                       static_cast<const float *>(u2), c.nx, c.ny, c.nz, static_cast<int *>(ws));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_check(e, "shell comparison launch");
+
+    const void *fn = stream_fn(c.op, c.cx, c.cy);
+    SmArgs a;
+    a.nx = c.nx; a.ny = c.ny; a.nz = c.nz;
+    a.sz = (long long)c.nx * c.ny;
+    a.core_h = c.wy * c.cy - 4;
+    a.tiles_y = (c.ny + a.core_h - 1) / a.core_h;
+    a.zchunk = c.zchunk;
+    a.ntiles = a.tiles_y * ((c.nz + c.zchunk - 1) / c.zchunk);
+    a.iters = c.iters;
+    a.n1 = c.iters % 4;
+    a.n2 = (c.iters - a.n1) / 2;
+    a.shell_eq = static_cast<const int *>(ws);
+    const int threads = 64 * c.wy;
+    e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds);
+    if (e != hipSuccess) return hip_check(e, "hipFuncSetAttribute(stream LDS)");
+    int per_cu = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, c.lds);
+    if (e != hipSuccess) return hip_check(e, "occupancy query");
+    if (per_cu < 1) return fail(ST_ELDS, "streaming kernel does not fit on a CU");
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    int grid = per_cu * cus;
+    if (grid > a.ntiles) grid = a.ntiles;
+    float kf = c.op == OP_THESIS ? (float)(2 * c.dims * c.r) : (float)c.weight;
+    void *args[] = {&u1, &u2, &a, &kf};
+    e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, c.lds, s);
+    if (e != hipSuccess) return hip_check(e, "hipLaunchCooperativeKernel(stream)");
+    return ST_OK;
+}
+
+}  // namespace st
