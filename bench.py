@@ -35,7 +35,7 @@ def algorithmic_bytes_per_sweep(n=N, r=RADIUS, es=4):
 
 
 def cpu_baseline(budget_s: float = 12.0):
-    """Time the CPU oracle on sweeps of the same 512^3 field until the
+    """Time the CPU oracle on sweeps of the same field (FP32) until the
     budget is used (at least 2 sweeps); report GCells/s."""
     import numpy as np
 
@@ -57,7 +57,7 @@ def cpu_baseline(budget_s: float = 12.0):
     cells = (N - 2 * RADIUS) ** 3 * sweeps
     return {"value": round(cells / el / 1e9, 4), "unit": "GCells/s", "cores": threads,
             "kind": "port",
-            "sample": f"{sweeps} sweeps of the 512^3 R=1 FP32 JACOBI field "
+            "sample": f"{sweeps} sweeps of the {N}^3 R={RADIUS} FP32 JACOBI field "
                       f"({el:.1f} s, oracle/stencil_oracle.c, OpenMP {threads} threads)"}
 
 
@@ -180,6 +180,7 @@ def probe(st, np):
 
 
 def main():
+    global N, RADIUS, SWEEPS
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -200,6 +201,10 @@ def main():
                     help="short run for counter collection: one streaming copy of a 512^3 "
                          "field, 2 sweeps of each single-sweep family and of the LDS-ring "
                          "COOP form, then one 100-sweep launch of the streaming COOP form")
+    ap.add_argument("--dim", type=int, default=N, help="grid edge (default: the 512^3 headline)")
+    ap.add_argument("--radius", type=int, default=RADIUS)
+    ap.add_argument("--sweeps", type=int, default=SWEEPS, help="sweeps per step (even)")
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
     ap.add_argument("--init", default="seeded", choices=["seeded", "reference"],
                     help="seeded: the same boundary in both buffers (two sweeps fuse per pass "
                          "in the streaming form); reference: u2 zero (different shells, one "
@@ -229,7 +234,13 @@ def main():
         bench_mixed(st, np, args)
         return
 
-    kw = dict(dim=N, radius=RADIUS, iterations=SWEEPS, op=st.OP_JACOBI, dtype=st.F32,
+    headline = (args.dim, args.radius, args.sweeps, args.dtype) == (N, RADIUS, SWEEPS, "f32")
+    if args.sweeps % 2:
+        raise SystemExit("--sweeps must be even (the state stays in the first buffer)")
+    N, RADIUS, SWEEPS = args.dim, args.radius, args.sweeps
+    f64 = args.dtype == "f64"
+    es = 8 if f64 else 4
+    kw = dict(dim=N, radius=RADIUS, iterations=SWEEPS, op=st.OP_JACOBI, dtype=st.F64 if f64 else st.F32,
               variant=args.variant, heuristic=1, temporal_steps=args.temporal_steps)
     if args.block:
         bx, by, bz = (int(v) for v in args.block.split(","))
@@ -240,8 +251,8 @@ def main():
     p = st.StencilParams.make(**kw)
     cfg = st.launch_config(p)
     stream = st.Stream()
-    d1 = st.DeviceArray(p.shape, np.float32)
-    d2 = st.DeviceArray(p.shape, np.float32)
+    d1 = st.DeviceArray(p.shape, np.float64 if f64 else np.float32)
+    d2 = st.DeviceArray(p.shape, np.float64 if f64 else np.float32)
 
     # The field is initialised once; every step advances it by 100 more
     # sweeps (an even count, so the state stays in d1).  The work per sweep
@@ -268,13 +279,14 @@ def main():
     cells = (N - 2 * RADIUS) ** 3 * SWEEPS * args.steps
     gcells = cells / wall / 1e9
     launches = cfg["launches"]
-    bytes_sweep = algorithmic_bytes_per_sweep()
+    bytes_sweep = algorithmic_bytes_per_sweep(N, RADIUS, es)
     bytes_launch = bytes_sweep * SWEEPS / launches
     avg_launch_s = sweep_total_s / (launches * args.steps)
     achieved = bytes_launch / avg_launch_s / 1e9
     variant_name = {v: k for k, v in st.VARIANTS.items()}[cfg["variant"]]
     out = {
-        "metric": "GCells/s & %HBM roofline (512^3 r=1)",
+        "metric": "GCells/s & %HBM roofline (512^3 r=1)" if headline
+                  else f"GCells/s & %HBM roofline ({N}^3 r={RADIUS} {args.dtype})",
         "value": round(gcells, 3),
         "unit": "GCells/s",
         "n_gpus": 1,
@@ -285,15 +297,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": args.dtype,
         "data": ("synthetic: seeded uniform [0,1) boundary shell, zero interior" if args.init == "seeded"
                  else "synthetic: reference initial condition (u1 shell 1, u2 zero)"),
-        "config": {"workload": "3D 7-point JACOBI u'=(1/6)*sum of 6 neighbours, 512^3 FP32, "
-                               "radius 1, 100 sweeps per step",
+        "config": {"workload": (f"3D {6 * RADIUS + 1}-point JACOBI u'=(1/{6 * RADIUS})*sum of "
+                                f"{6 * RADIUS} neighbours, {N}^3 {args.dtype.upper()}, radius {RADIUS}, "
+                                f"{SWEEPS} sweeps per step"),
                    "variant": variant_name, "launch": cfg, "device": st.device_name(0)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic(variant_name),
+                     "traffic": load_traffic(variant_name) if headline else None,
                      "bytes_per_launch_algorithmic": bytes_launch,
                      "avg_launch_ms": round(avg_launch_s * 1e3, 5),
                      "note": "temporal-blocked launches advance several sweeps per HBM pass, "

@@ -92,7 +92,11 @@ def head(x, w, b):
 
 
 def features(u, C=8):
-    """(N, n, n, n) field with filled shell -> (feat (N,n,n,n,C), base)."""
+    """(N, n, n, n) field with filled shell -> (feat (N,n,n,n,C), base).
+
+    Channels: transfinite blend of the faces, the shell value (zero
+    inside), x, y, z in [0, 1], twice the distance to the nearest face,
+    then zeros."""
     N, n = u.shape[0], u.shape[1]
     s = np.arange(n) / (n - 1)
     fx = (1 - s)[None, None, None, :] * u[:, :, :, :1] + s[None, None, None, :] * u[:, :, :, -1:]
@@ -104,6 +108,16 @@ def features(u, C=8):
     feat = np.zeros(u.shape + (C,))
     feat[..., 0] = blend
     feat[..., 1] = np.where(shell, u, 0.0)
+    # coordinates (float32 as the kernel computes them) and the distance to
+    # the nearest face, scaled to [0, 1]
+    sf = (np.arange(n, dtype=np.float32) * np.float32(1.0 / (n - 1))).astype(np.float64)
+    cz, cy, cx = np.meshgrid(sf, sf, sf, indexing="ij")
+    feat[..., 2] = cx
+    feat[..., 3] = cy
+    feat[..., 4] = cz
+    d = np.minimum(np.minimum(np.minimum(cx, 1 - cx), np.minimum(cy, 1 - cy)),
+                   np.minimum(cz, 1 - cz))
+    feat[..., 5] = 2 * d
     base = np.where(shell, u, blend)
     return feat, base
 

@@ -384,3 +384,51 @@ def test_coop_stream_form_rejects_unsupported(gpu):
                                   iterations=4, **kw)
         with pytest.raises(st.StencilError):
             st.check_early_exit(p)
+
+
+# SMEM_REGISTER row-streaming form (one sweep per launch, radius 1..4)
+ROWS_CASES = [
+    ((64, 40, 37), "f32", (1, 2, 3, 4), {}),
+    ((384, 30, 24), "f32", (1, 2, 3, 4), {}),       # 8 cells per lane, half-used second quad
+    ((260, 21, 17), "f32", (2, 4), {}),
+    ((12, 11, 13), "f32", (1, 3), {}),
+    ((128, 50, 45), "f32", (1, 4), {"block_y": 12, "block_z": 11, "heuristic": 0}),
+    ((64, 33, 29), "f64", (1, 2, 3, 4), {}),
+    ((300, 19, 18), "f64", (1, 2), {}),
+]
+
+
+@pytest.mark.parametrize("shape,dt,radii,knobs", ROWS_CASES, ids=str)
+@pytest.mark.parametrize("op", [0, 1])
+def test_smem_register_rows_form(gpu, oracle, shape, dt, radii, knobs, op):
+    st = gpu
+    nx, ny, nz = shape
+    npdt = np.float64 if dt == "f64" else np.float32
+    for r in radii:
+        _announce("rows form", shape, dt, r, op, knobs)
+        u1 = oracle.init_seeded(nx, ny, nz, 3, r, 0x5EED0008 + r, dtype=npdt)
+        rng = np.random.default_rng(nx + r)
+        u1 = np.where(u1 == 0, rng.random(u1.shape), u1).astype(npdt)
+        u2 = np.zeros_like(u1) if op == 0 else u1.copy()
+        kw = dict(heuristic=1)
+        kw.update(knobs)
+        if kw["heuristic"] == 0:
+            kw["block_x"] = nx
+        for its in (1, 3):
+            want = oracle.run(u1.copy(), u2.copy(), 3, r, op, its).copy()
+            p = st.StencilParams.make(nx=nx, ny=ny, nz=nz, dimensions=3, radius=r, iterations=its, op=op,
+                                      dtype=st.F64 if dt == "f64" else st.F32, variant="smem_register",
+                                      temporal_form=2, **kw)
+            got, in_u2 = run_gpu(st, p, u1, u2)
+            assert in_u2 == bool(its % 2)
+            assert_bitwise(got, want, f"rows form {shape} {dt} r={r} op={op} its={its}")
+
+
+def test_smem_register_rows_form_rejects_unsupported(gpu):
+    st = gpu
+    for kw in (dict(nx=62, ny=40, nz=40), dict(nx=520, ny=40, nz=40), dict(dim=40, radius=5),
+               dict(nx=384, ny=20, nz=20, radius=3, dtype=st.F64), dict(nx=64, ny=40, nz=1, dimensions=2)):
+        p = st.StencilParams.make(variant="smem_register", temporal_form=2, heuristic=1, op=1,
+                                  iterations=2, **kw)
+        with pytest.raises(st.StencilError):
+            st.check_early_exit(p)

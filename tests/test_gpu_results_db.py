@@ -1,0 +1,29 @@
+"""The result databases filled on the GPU through the C ABI."""
+import json
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_evaluate_and_autotune_on_device(gpu, tmp_path):
+    from thesis_amd import results_db as R
+    print("[case] results_db evaluate 3D 64^3", flush=True)
+    out = str(tmp_path / "results.json")
+    R.evaluate_stencils(out, [3], {3: [64]}, ["base", "smem", "smem_padded"], [1, 2], [1, 2], [8],
+                        host="box", repeat=3)
+    db = json.load(open(out))
+    for v in ("1_gpus_base", "1_gpus_smem_unroll_2", "1_gpus_smem_padded"):
+        for r in ("1", "2"):
+            t = db["3"]["64"][v][r]["8"]["box"]["heuristic"]["time"]
+            assert len(t) == 3 and all(x > 0 for x in t), (v, r, t)
+    print("[case] results_db autotune 3D 64^3", flush=True)
+    tune = str(tmp_path / "results_autotune.json")
+    values = {"BLOCK_X": ["32", "64"], "BLOCK_Y": ["1", "2", "4"], "BLOCK_Z": ["1", "2"]}
+    R.autotune_stencils(tune, [3], {3: [64]}, ["smem"], [1], [1], iterations=8, repeat=3, values=values)
+    tdb = json.load(open(tune))
+    e = tdb["3"]["64"]["smem"]["1"]
+    assert e["BLOCK_X"] * e["BLOCK_Y"] * e["BLOCK_Z"] in (64, 128, 256, 512)
+    R.evaluate_stencils(out, [3], {3: [64]}, ["smem"], [1], [1], [8], host="box", repeat=2, tune_db=tune)
+    db = json.load(open(out))
+    assert len(db["3"]["64"]["1_gpus_smem"]["1"]["8"]["box"]["autotune"]["time"]) == 2

@@ -11,68 +11,10 @@
 // defect is recomputed in FP32 from the new x (the FP32-residual defect
 // correction); 16-bit rounding then only ever applies to increments.
 #include "st_device.hpp"
+#include "st_half.hpp"
 #include "st_internal.hpp"
 
 namespace st {
-
-// Round-to-nearest-even FP32 -> BF16 / FP16 and back, in integer
-// arithmetic identical to the oracle's, so results are bit-exact.
-__device__ __forceinline__ uint16_t to_bf16(float f)
-{
-    uint32_t x = __float_as_uint(f);
-    if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40);
-    const uint32_t lsb = (x >> 16) & 1u;
-    x += 0x7fffu + lsb;
-    return (uint16_t)(x >> 16);
-}
-
-__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-
-__device__ __forceinline__ uint16_t to_f16(float f)
-{
-    const uint32_t x = __float_as_uint(f);
-    const uint32_t sign = (x >> 16) & 0x8000u;
-    const uint32_t ax = x & 0x7fffffffu;
-    if (ax > 0x7f800000u) return (uint16_t)(sign | 0x7e00u);
-    if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);
-    if (ax < 0x38800000u) {
-        if (ax < 0x33000000u) return (uint16_t)sign;
-        const uint32_t e = ax >> 23;
-        const uint32_t m = (ax & 0x7fffffu) | 0x800000u;
-        const uint32_t shift = 126u - e;
-        uint32_t q = m >> shift;
-        const uint32_t rem = m & ((1u << shift) - 1u);
-        const uint32_t half = 1u << (shift - 1u);
-        if (rem > half || (rem == half && (q & 1u))) q++;
-        return (uint16_t)(sign | q);
-    }
-    const uint32_t e = (ax >> 23) - 112u;
-    const uint32_t m = ax & 0x7fffffu;
-    uint32_t q = (e << 10) | (m >> 13);
-    const uint32_t rem = m & 0x1fffu;
-    if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) q++;
-    return (uint16_t)(sign | q);
-}
-
-__device__ __forceinline__ float from_f16(uint16_t h)
-{
-    const uint32_t sign = ((uint32_t)h & 0x8000u) << 16;
-    const uint32_t e = (h >> 10) & 0x1fu;
-    const uint32_t m = h & 0x3ffu;
-    uint32_t x;
-    if (e == 0) {
-        if (m == 0) x = sign;
-        else x = __float_as_uint((float)m * (1.0f / 16777216.0f)) | sign;
-    } else if (e == 31) {
-        x = sign | 0x7f800000u | (m << 13);
-    } else {
-        x = sign | ((e + 112u) << 23) | (m << 13);
-    }
-    return __uint_as_float(x);
-}
-
-template <bool F16> __device__ __forceinline__ uint16_t rnd16(float f) { return F16 ? to_f16(f) : to_bf16(f); }
-template <bool F16> __device__ __forceinline__ float wid16(uint16_t h) { return F16 ? from_f16(h) : from_bf16(h); }
 
 struct MixArgs {
     Geom g;
@@ -151,6 +93,14 @@ __global__ __launch_bounds__(256) void k_mx_sweep(const uint16_t *__restrict__ e
     }
 }
 
+// v <- 0 over n 16-bit values (a kernel rather than a memset, so that the
+// sequence can be captured in a graph and replayed)
+__global__ void k_fill16(uint16_t *__restrict__ v, long long n)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += stride) v[i] = 0;
+}
+
 // x += widen(e); e <- 0 (whole grid, elementwise).
 template <bool F16>
 __global__ __launch_bounds__(256) void k_mx_fold(float *__restrict__ x, uint16_t *__restrict__ e,
@@ -191,22 +141,39 @@ static st_status mixed_impl(const Cfg &c, float *x, uint16_t *e1, uint16_t *e2, 
         if (b > 32768) b = 32768;
         return dim3((unsigned)(b < 1 ? 1 : b));
     };
-    hipError_t err;
-    if ((err = hipMemsetAsync(e1, 0, n * 2, s)) != hipSuccess) return hip_check(err, "memset e1");
-    if ((err = hipMemsetAsync(e2, 0, n * 2, s)) != hipSuccess) return hip_check(err, "memset e2");
-    if ((err = hipMemsetAsync(d, 0, n * 2, s)) != hipSuccess) return hip_check(err, "memset d");
-    hipLaunchKernelGGL(k_mx_defect<F16>, blocks(ni), dim3(256), 0, s, x, d, a);
+    hipLaunchKernelGGL(k_fill16, blocks(n), dim3(256), 0, s, e1, n);
+    hipLaunchKernelGGL(k_fill16, blocks(n), dim3(256), 0, s, e2, n);
+    hipLaunchKernelGGL(k_fill16, blocks(n), dim3(256), 0, s, d, n);
+    // 3D grids the row-streaming kernels take (FP32 arithmetic on 16-bit
+    // loads, same operation order); the rest use the per-cell kernels
+    Cfg rc = c;
+    rc.dtype = ST_F32;
+    rc.op = OP_JACOBI;
+    rc.variant = ST_V_SMEM_REGISTER;
+    rc.heuristic = 1;
+    const bool rows = c.dims == 3 && rows_mixed_eligible(c) && resolve_rows(rc) == ST_OK;
+    st_status st = ST_OK;
+    auto defect = [&]() {
+        if (rows) st = launch_rows_mixed(rc, 1, F16, x, d, nullptr, s);
+        else hipLaunchKernelGGL(k_mx_defect<F16>, blocks(ni), dim3(256), 0, s, x, d, a);
+    };
+    auto sweep = [&](const uint16_t *ein, uint16_t *eout) {
+        if (rows) st = launch_rows_mixed(rc, 0, F16, ein, eout, d, s);
+        else hipLaunchKernelGGL(k_mx_sweep<F16>, blocks(ni), dim3(256), 0, s, ein, d, eout, a);
+    };
+    defect();
     uint16_t *ea = e1, *eb = e2;
-    for (int it = 1; it <= c.iters; ++it) {
-        hipLaunchKernelGGL(k_mx_sweep<F16>, blocks(ni), dim3(256), 0, s, ea, d, eb, a);
+    for (int it = 1; it <= c.iters && st == ST_OK; ++it) {
+        sweep(ea, eb);
         uint16_t *t = ea; ea = eb; eb = t;
         if (it % c.restart == 0 && it < c.iters) {
             // eb's interior is overwritten by the next sweep and its shell is
             // still 0, so only the live correction needs clearing.
             hipLaunchKernelGGL(k_mx_fold<F16>, blocks(n), dim3(256), 0, s, x, ea, n);
-            hipLaunchKernelGGL(k_mx_defect<F16>, blocks(ni), dim3(256), 0, s, x, d, a);
+            defect();
         }
     }
+    if (st != ST_OK) return st;
     hipLaunchKernelGGL(k_mx_final<F16>, blocks(n), dim3(256), 0, s, x, ea, u, n);
     return hip_check(hipGetLastError(), "mixed launch");
 }

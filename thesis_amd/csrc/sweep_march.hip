@@ -117,6 +117,16 @@ static const MarchEntry *best_march(int d, int f64, int r, int cx, int ty)
 
 st_status resolve_march(Cfg &c)
 {
+    // form: 1 register march, 2 row streaming; the library's choice is the
+    // row-streaming form for radius 2..4 and for FP64 where it applies
+    {
+        const bool heur0 = c.heuristic || c.bx <= 0;
+        if (c.form == 2 || (c.form == 0 && heur0 && rows_eligible(c) && (c.r >= 2 || c.dtype == ST_F64))) {
+            if (heur0) c.heuristic = 1;
+            return resolve_rows(c);
+        }
+        if (c.form > 2) return fail(ST_EINVAL, "SMEM_REGISTER forms are 0 (auto), 1 (march), 2 (rows)");
+    }
     const int f64 = c.dtype == ST_F64;
     int cx = c.cx > 0 ? c.cx : 1;
     int ty = c.dims == 3 ? (c.cy > 0 ? c.cy : 0) : 1;
@@ -173,6 +183,7 @@ st_status resolve_march(Cfg &c)
 
 st_status launch_march(const Cfg &c, const void *in, void *out, hipStream_t s)
 {
+    if (c.kind == 4) return launch_rows(c, in, out, s);
     const MarchEntry *e = find_march(c.dims, c.dtype == ST_F64, c.r, c.cx, c.cy);
     if (!e) return fail(ST_EUNSUPPORTED, "no SMEM_REGISTER kernel for r=%d cx=%d ty=%d", c.r, c.cx, c.cy);
     const int es = c.dtype == ST_F64 ? 8 : 4;

@@ -1,0 +1,519 @@
+// SMEM_REGISTER family, row-streaming form: one sweep, radius 1..4, FP32 or
+// FP64, whole grid rows per wave.
+//
+// The reference's smem_register kernels keep an xy tile in shared memory and
+// each thread's z column in registers (src/stencil_kernel_smem_register.cu,
+// src/prefetch_smem.cu:151-258).  Here:
+//   * a workgroup of WY waves owns a frame of WY consecutive rows (one row
+//     per wave, the whole row: each lane holds CX = 4*H cells as H quads of
+//     4 consecutive columns 256 apart, so every 16-byte access of a wave
+//     covers 1 KiB contiguously) and marches it along z over a chunk;
+//   * per plane, every wave writes its row to an LDS plane, and waves
+//     0..2R-1 also load and write one of the frame's 2R halo rows (R above,
+//     R below), so y neighbours come from LDS with one barrier per plane and
+//     no row is read twice by the workgroup; the LDS plane is double
+//     buffered;
+//   * x neighbours at distance d <= 4 lie in the lane's own quad or in the
+//     adjacent lane's quad: wave-wide DPP rotates (`wave_ror:1`,
+//     `wave_rol:1`), with lane 0 / lane 63 taking the neighbouring quad of
+//     the other end of the wave;
+//   * z: a ring of the R previous planes and R running partial sums.  When
+//     plane m arrives its x and y terms and the ring (planes m-R..m-1) start
+//     P_m in the reference's order (i-prev d=R..1, i-next 1..R, j-prev,
+//     j-next, k-prev), and m is added to P_{m-1}..P_{m-R} as their next
+//     k-next term; P_{m-R} is then complete.
+// Level 0 of the next plane (own row and halo row) is prefetched into
+// registers before the current plane is processed.  Only interior cells are
+// stored; the output's shell is untouched.
+//
+// Two more modes serve the 16-bit storage scheme of mixed.hip (JACOBI, FP32
+// arithmetic, T = float): the correction sweep reads 16-bit e, widens it,
+// and writes round16(widen(d) + w*sum) (d read at the output cell), the
+// defect reads FP32 x and writes round16(w*sum - x).
+#include "st_device.hpp"
+#include "st_half.hpp"
+#include "st_internal.hpp"
+
+namespace st {
+
+struct RowsArgs {
+    int nx, ny, nz;
+    long long sz;   // nx*ny
+    int wy;         // core rows per frame (= waves per workgroup)
+    int tiles_y, zchunk, ntiles;
+    int nt;         // non-temporal stores
+};
+
+template <typename T> struct Quad;
+template <> struct Quad<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct Quad<double> { typedef double type __attribute__((ext_vector_type(4))); };
+
+template <typename T>
+__device__ __forceinline__ T dpp_prev(T v)  // lane i <- lane i-1 (rotating)
+{
+    if constexpr (sizeof(T) == 4) {
+        return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x13C, 0xF, 0xF, false));
+    } else {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x13C, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x13C, 0xF, 0xF, false);
+        return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T dpp_next(T v)  // lane i <- lane i+1 (rotating)
+{
+    if constexpr (sizeof(T) == 4) {
+        return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134, 0xF, 0xF, false));
+    } else {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x134, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x134, 0xF, 0xF, false);
+        return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    }
+}
+
+// VGPRs a lane needs (fitted to the compiler's allocation: ring, partials,
+// prefetched rows, DPP copies of the neighbouring quads, temporaries), and
+// the workgroup size that leaves that many: 1024 threads leave 128, 768
+// leave 168, 512 leave 256.  Shapes above 250 are not built.
+enum : int { RM_SWEEP = 0, RM_CORR = 1, RM_DEFECT = 2 };
+
+template <int R, int CX, typename T, int MODE = RM_SWEEP>
+constexpr int rows_vgprs()
+{
+    return (9 * R + 3) * CX * (int)(sizeof(T) / 4) / 2 + 25 + (MODE == RM_CORR ? CX : 0);
+}
+
+template <int R, int CX, typename T, int MODE = RM_SWEEP>
+constexpr bool rows_supported() { return rows_vgprs<R, CX, T, MODE>() <= 250; }
+
+template <int R, int CX, typename T, int MODE = RM_SWEEP>
+constexpr int rows_max_threads()
+{
+    return rows_vgprs<R, CX, T, MODE>() <= 120 ? 1024 : rows_vgprs<R, CX, T, MODE>() <= 160 ? 768 : 512;
+}
+
+template <int OP, int R, int CX, typename T, int MODE = RM_SWEEP, bool F16 = false>
+__global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE>())) void k_rows(
+    const void *__restrict__ in_, void *__restrict__ out_, const uint16_t *__restrict__ aux, RowsArgs a,
+    T kc)
+{
+    static_assert(R >= 1 && R <= 4, "x neighbours come from the adjacent lane's quad");
+    static_assert(MODE == RM_SWEEP || sizeof(T) == 4, "the 16-bit modes compute in FP32");
+    constexpr int H = CX / 4;
+    typedef typename Quad<T>::type QV;
+    typedef uint16_t U4 __attribute__((ext_vector_type(4)));
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    QV *lds = reinterpret_cast<QV *>(lds_raw);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wy = a.wy;
+    const int lrows = wy + 2 * R;  // LDS rows per buffer: R halo, wy core, R halo
+    const bool first = lane == 0, last = lane == 63;
+
+    int xo[H];
+    bool act[H], intq[H];
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+        const int x = 256 * h + 4 * lane;
+        act[h] = x < a.nx;
+        xo[h] = act[h] ? x : a.nx - 4;  // quads past the row reload its last cells
+        intq[h] = act[h] && x >= R && x + 4 <= a.nx - R;
+    }
+    const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+
+    for (int tile = (int)bid; tile < a.ntiles; tile += (int)gridDim.x) {
+        const int tyi = tile % a.tiles_y, tzi = tile / a.tiles_y;
+        const int fy0 = tyi * wy;
+        const int gy = fy0 + w;  // own (core) row
+        const int own_off = min(gy, a.ny - 1) * a.nx;
+        const bool store_row = gy >= R && gy < a.ny - R;
+        // halo row of this wave (waves 0..2R-1): frame rows -R..-1 and wy..wy+R-1
+        const bool has_halo = w < 2 * R;
+        const int hrow = w < R ? w : wy + w;                        // LDS row
+        const int hgy = w < R ? fy0 - R + w : fy0 + wy + (w - R);   // grid row
+        const int halo_off = min(max(hgy, 0), a.ny - 1) * a.nx;
+        const int z0 = tzi * a.zchunk, z1 = min(z0 + a.zchunk, a.nz);
+
+        auto load_row = [&](int k, int off, T (&d)[CX]) {
+            k = min(max(k, 0), a.nz - 1);
+            if constexpr (MODE == RM_CORR) {
+                const uint16_t *p = static_cast<const uint16_t *>(in_) + (long long)k * a.sz + off;
+#pragma unroll
+                for (int h = 0; h < H; ++h) {
+                    const U4 q = *reinterpret_cast<const U4 *>(p + xo[h]);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) d[4 * h + j] = wid16<F16>(q[j]);
+                }
+            } else {
+                const T *p = static_cast<const T *>(in_) + (long long)k * a.sz + off;
+#pragma unroll
+                for (int h = 0; h < H; ++h) {
+                    const QV q = *reinterpret_cast<const QV *>(p + xo[h]);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) d[4 * h + j] = q[j];
+                }
+            }
+        };
+        const int k_out_lo = max(z0, R), k_out_hi = min(z1, a.nz - R);
+
+        T ring[R][CX], part[R][CX], pf[CX], ph[CX];
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int c = 0; c < CX; ++c) ring[i][c] = part[i][c] = (T)0;
+        const int kin_lo = z0 - R, kin_hi = z1 + R;
+        load_row(kin_lo, own_off, pf);
+        if (has_halo) load_row(kin_lo, halo_off, ph);
+
+        for (int m = kin_lo; m < kin_hi; ++m) {
+            T nw[CX];
+#pragma unroll
+            for (int c = 0; c < CX; ++c) nw[c] = pf[c];
+            QV *buf = lds + (size_t)((m & 1) * lrows) * H * 64;
+#pragma unroll
+            for (int h = 0; h < H; ++h) {
+                QV q;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) q[j] = nw[4 * h + j];
+                buf[((R + w) * H + h) * 64 + lane] = q;
+            }
+            if (has_halo) {
+#pragma unroll
+                for (int h = 0; h < H; ++h) {
+                    QV q;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) q[j] = ph[4 * h + j];
+                    buf[(hrow * H + h) * 64 + lane] = q;
+                }
+            }
+            if (m + 1 < kin_hi) {
+                load_row(m + 1, own_off, pf);
+                if (has_halo) load_row(m + 1, halo_off, ph);
+            }
+            // correction sweep: the defect at the cell this step completes
+            // (plane m-R), loaded now and used after the barrier
+            const int k = m - R;
+            const bool out_plane = store_row && k >= k_out_lo && k < k_out_hi;
+            U4 dq[H];
+            if constexpr (MODE == RM_CORR) {
+                if (out_plane) {
+                    const uint16_t *p = aux + (long long)k * a.sz + own_off;
+#pragma unroll
+                    for (int h = 0; h < H; ++h) dq[h] = *reinterpret_cast<const U4 *>(p + xo[h]);
+                }
+            }
+            __syncthreads();
+
+            // x terms of plane m: own quad, else the adjacent lane's quad
+            T pq[H][4], nq[H][4];
+#pragma unroll
+            for (int h = 0; h < H; ++h)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    pq[h][j] = dpp_prev(nw[4 * h + j]);
+                    nq[h][j] = dpp_next(nw[4 * h + j]);
+                }
+            T s[CX];
+#pragma unroll
+            for (int h = 0; h < H; ++h) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    T u = (T)0;
+#pragma unroll
+                    for (int d = R; d >= 1; --d) {
+                        T v;
+                        if (j - d >= 0) {
+                            v = nw[4 * h + j - d];
+                        } else {
+                            const T mine = pq[h][4 + j - d];
+                            const T other = h > 0 ? pq[h - 1][4 + j - d] : mine;
+                            v = first ? other : mine;
+                        }
+                        u += v;
+                    }
+#pragma unroll
+                    for (int d = 1; d <= R; ++d) {
+                        T v;
+                        if (j + d < 4) {
+                            v = nw[4 * h + j + d];
+                        } else {
+                            const T mine = nq[h][j + d - 4];
+                            const T other = h < H - 1 ? nq[h + 1][j + d - 4] : mine;
+                            v = last ? other : mine;
+                        }
+                        u += v;
+                    }
+                    s[4 * h + j] = u;
+                }
+            }
+            // y terms from the LDS plane: rows R+w-d (d = R..1), then R+w+d
+#pragma unroll
+            for (int d = R; d >= 1; --d) {
+#pragma unroll
+                for (int h = 0; h < H; ++h) {
+                    const QV q = buf[((R + w - d) * H + h) * 64 + lane];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) s[4 * h + j] += q[j];
+                }
+            }
+#pragma unroll
+            for (int d = 1; d <= R; ++d) {
+#pragma unroll
+                for (int h = 0; h < H; ++h) {
+                    const QV q = buf[((R + w + d) * H + h) * 64 + lane];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) s[4 * h + j] += q[j];
+                }
+            }
+            // k-prev terms: ring[0] = plane m-R ... ring[R-1] = plane m-1
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+#pragma unroll
+                for (int c = 0; c < CX; ++c) s[c] += ring[i][c];
+            // plane m is the next k-next term of P_{m-R} .. P_{m-1}
+            // (part[0] = P_{m-R}, which it completes)
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+#pragma unroll
+                for (int c = 0; c < CX; ++c) part[i][c] += nw[c];
+            // P_{m-R} is complete: plane k = m-R, centre ring[0]
+            if (out_plane) {
+                if constexpr (MODE == RM_SWEEP) {
+                    T res[CX];
+#pragma unroll
+                    for (int c = 0; c < CX; ++c) res[c] = finish<OP>(part[0][c], ring[0][c], kc);
+                    T *row = static_cast<T *>(out_) + (long long)k * a.sz + (long long)gy * a.nx;
+#pragma unroll
+                    for (int h = 0; h < H; ++h) {
+                        if (intq[h]) {
+                            QV q;
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) q[j] = res[4 * h + j];
+                            if (a.nt) __builtin_nontemporal_store(q, reinterpret_cast<QV *>(row + xo[h]));
+                            else *reinterpret_cast<QV *>(row + xo[h]) = q;
+                        } else if (act[h]) {
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+                                if (interior1(xo[h] + j, a.nx, R)) row[xo[h] + j] = res[4 * h + j];
+                        }
+                    }
+                } else {
+                    // 16-bit output: correction round16(widen(d) + w*sum),
+                    // defect round16(w*sum - x)
+                    U4 res[H];
+#pragma unroll
+                    for (int h = 0; h < H; ++h)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const float sw = part[0][4 * h + j] * kc;
+                            if constexpr (MODE == RM_CORR) res[h][j] = rnd16<F16>(wid16<F16>(dq[h][j]) + sw);
+                            else res[h][j] = rnd16<F16>(sw - ring[0][4 * h + j]);
+                        }
+                    uint16_t *row = static_cast<uint16_t *>(out_) + (long long)k * a.sz + (long long)gy * a.nx;
+#pragma unroll
+                    for (int h = 0; h < H; ++h) {
+                        if (intq[h]) {
+                            *reinterpret_cast<U4 *>(row + xo[h]) = res[h];
+                        } else if (act[h]) {
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+                                if (interior1(xo[h] + j, a.nx, R)) row[xo[h] + j] = res[h][j];
+                        }
+                    }
+                }
+            }
+            // shift: ring <- planes m-R+1..m, partials <- P_{m-R+1}..P_m
+#pragma unroll
+            for (int i = 0; i + 1 < R; ++i)
+#pragma unroll
+                for (int c = 0; c < CX; ++c) {
+                    ring[i][c] = ring[i + 1][c];
+                    part[i][c] = part[i + 1][c];
+                }
+#pragma unroll
+            for (int c = 0; c < CX; ++c) {
+                ring[R - 1][c] = nw[c];
+                part[R - 1][c] = s[c];
+            }
+        }
+        __syncthreads();  // the next tile rewrites the LDS planes
+    }
+}
+
+// ---------------------------------------------------------------- host side
+
+static int rows_cx(const Cfg &c)
+{
+    if (c.nx % 4 || c.nx > 512) return 0;
+    return c.nx <= 256 ? 4 : 8;
+}
+
+
+template <int R, int CX, typename T, int MODE, bool F16>
+static const void *rows_fn_t(int op)
+{
+    if constexpr (!rows_supported<R, CX, T, MODE>()) {
+        return nullptr;
+    } else if constexpr (MODE != RM_SWEEP) {
+        return reinterpret_cast<const void *>(&k_rows<OP_JACOBI, R, CX, T, MODE, F16>);
+    } else {
+        return op == OP_THESIS ? reinterpret_cast<const void *>(&k_rows<OP_THESIS, R, CX, T>)
+                               : reinterpret_cast<const void *>(&k_rows<OP_JACOBI, R, CX, T>);
+    }
+}
+
+template <int R, typename T, int MODE, bool F16>
+static const void *rows_fn_r(int op, int cx, int *threads)
+{
+    if (cx == 8) {
+        *threads = rows_max_threads<R, 8, T, MODE>();
+        return rows_fn_t<R, 8, T, MODE, F16>(op);
+    }
+    *threads = rows_max_threads<R, 4, T, MODE>();
+    return rows_fn_t<R, 4, T, MODE, F16>(op);
+}
+
+template <typename T, int MODE, bool F16>
+static const void *rows_fn_d(int op, int r, int cx, int *threads)
+{
+    switch (r) {
+    case 1: return rows_fn_r<1, T, MODE, F16>(op, cx, threads);
+    case 2: return rows_fn_r<2, T, MODE, F16>(op, cx, threads);
+    case 3: return rows_fn_r<3, T, MODE, F16>(op, cx, threads);
+    default: return rows_fn_r<4, T, MODE, F16>(op, cx, threads);
+    }
+}
+
+// mode RM_SWEEP: c.dtype F32/F64; RM_CORR / RM_DEFECT: FP32 arithmetic on
+// 16-bit storage (f16 selects FP16, else BF16)
+static const void *rows_fn_mode(const Cfg &c, int mode, bool f16, int *threads)
+{
+    const int cx = rows_cx(c);
+    if (mode == RM_CORR)
+        return f16 ? rows_fn_d<float, RM_CORR, true>(c.op, c.r, cx, threads)
+                   : rows_fn_d<float, RM_CORR, false>(c.op, c.r, cx, threads);
+    if (mode == RM_DEFECT)
+        return f16 ? rows_fn_d<float, RM_DEFECT, true>(c.op, c.r, cx, threads)
+                   : rows_fn_d<float, RM_DEFECT, false>(c.op, c.r, cx, threads);
+    if (c.dtype == ST_F64) return rows_fn_d<double, RM_SWEEP, false>(c.op, c.r, cx, threads);
+    return rows_fn_d<float, RM_SWEEP, false>(c.op, c.r, cx, threads);
+}
+
+static const void *rows_fn(const Cfg &c, int *threads) { return rows_fn_mode(c, RM_SWEEP, false, threads); }
+
+bool rows_eligible(const Cfg &c)
+{
+    if (!((c.dtype == ST_F32 || c.dtype == ST_F64) && c.dims == 3 && c.r >= 1 && c.r <= 4 && c.nx >= 8 &&
+          rows_cx(c) != 0))
+        return false;
+    int threads = 0;
+    return rows_fn(c, &threads) != nullptr;
+}
+
+st_status resolve_rows(Cfg &c)
+{
+    if (!rows_eligible(c))
+        return fail(ST_EUNSUPPORTED, "row-streaming form needs a 3D F32/F64 grid, radius 1..4, nx a "
+                                     "multiple of 4 up to 512 (FP64 with nx > 256: radius 1..2)");
+    const int cx = rows_cx(c);
+    int threads = 0;
+    (void)rows_fn(c, &threads);
+    int wy = threads / 64;
+    if (!c.heuristic && c.by > 0) {
+        // block_y = rows per frame (waves per workgroup)
+        if (c.by < 2 * c.r || c.by > wy)
+            return fail(ST_EINVAL, "row-streaming form: block_y (rows per frame) must be %d..%d", 2 * c.r, wy);
+        wy = c.by;
+    } else if (wy > c.ny) {
+        wy = c.ny < 2 * c.r ? 2 * c.r : c.ny;
+    }
+    const int es = c.dtype == ST_F64 ? 8 : 4;
+    c.lds = 2 * (wy + 2 * c.r) * (cx / 4) * 64 * 4 * es;
+    if (c.lds > kMaxLds) return fail(ST_ELDS, "row-streaming form needs %d bytes of LDS", c.lds);
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 256;
+    const int tiles_y = (c.ny + wy - 1) / wy;
+    int chunk = (!c.heuristic && c.bz > 0) ? c.bz : 0;
+    if (chunk <= 0) {
+        // one tile per CU (the LDS planes allow one workgroup per CU)
+        int chunks = cus / tiles_y;
+        if (chunks < 1) chunks = 1;
+        chunk = (c.nz + chunks - 1) / chunks;
+        if (chunk < 2 * c.r) chunk = 2 * c.r;
+        if (chunk > c.nz) chunk = c.nz;
+    }
+    c.kind = 4;
+    c.cx = cx;
+    c.cy = 1;
+    c.wy = wy;
+    c.bx = c.nx;
+    c.by = wy;
+    c.bz = chunk;
+    c.zchunk = chunk;
+    c.gx = tiles_y * ((c.nz + chunk - 1) / chunk);
+    c.gy = c.gz = 1;
+    return ST_OK;
+}
+
+static st_status rows_launch(const Cfg &c, const void *fn, const void *in, void *out, const uint16_t *aux,
+                             hipStream_t s)
+{
+    RowsArgs a;
+    a.nx = c.nx; a.ny = c.ny; a.nz = c.nz;
+    a.sz = (long long)c.nx * c.ny;
+    a.wy = c.wy;
+    a.tiles_y = (c.ny + c.wy - 1) / c.wy;
+    a.zchunk = c.zchunk;
+    a.ntiles = a.tiles_y * ((c.nz + c.zchunk - 1) / c.zchunk);
+    a.nt = 1;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds);
+    if (e != hipSuccess) return hip_check(e, "hipFuncSetAttribute(rows LDS)");
+    const int nthreads = 64 * c.wy;
+    if (c.dtype == ST_F64) {
+        double kd = c.op == OP_THESIS ? (double)(2 * c.dims * c.r) : c.weight;
+        void *args[] = {&in, &out, &aux, &a, &kd};
+        e = hipLaunchKernel(fn, dim3(a.ntiles), dim3(nthreads), args, c.lds, s);
+    } else {
+        float kf = c.op == OP_THESIS ? (float)(2 * c.dims * c.r) : (float)c.weight;
+        void *args[] = {&in, &out, &aux, &a, &kf};
+        e = hipLaunchKernel(fn, dim3(a.ntiles), dim3(nthreads), args, c.lds, s);
+    }
+    if (e != hipSuccess) return hip_check(e, "rows launch");
+    return ST_OK;
+}
+
+st_status launch_rows(const Cfg &c, const void *in, void *out, hipStream_t s)
+{
+    int threads = 0;
+    const void *fn = rows_fn(c, &threads);
+    if (!fn) return fail(ST_EUNSUPPORTED, "row-streaming kernel missing");
+    return rows_launch(c, fn, in, out, nullptr, s);
+}
+
+bool rows_mixed_eligible(const Cfg &c)
+{
+    Cfg f = c;
+    f.dtype = ST_F32;
+    f.op = OP_JACOBI;
+    if (!rows_eligible(f)) return false;
+    int threads = 0;
+    return rows_fn_mode(f, RM_CORR, c.dtype == ST_F16, &threads) &&
+           rows_fn_mode(f, RM_DEFECT, c.dtype == ST_F16, &threads);
+}
+
+st_status launch_rows_mixed(const Cfg &rc, int defect, bool f16, const void *in, void *out, const uint16_t *d,
+                            hipStream_t s)
+{
+    int threads = 0;
+    const void *fn = rows_fn_mode(rc, defect ? RM_DEFECT : RM_CORR, f16, &threads);
+    if (!fn) return fail(ST_EUNSUPPORTED, "row-streaming mixed kernel missing");
+    if (64 * rc.wy > threads) return fail(ST_EINVAL, "row-streaming mixed kernel: frame too tall");
+    return rows_launch(rc, fn, in, out, d, s);
+}
+
+}  // namespace st

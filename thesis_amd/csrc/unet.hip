@@ -586,8 +586,18 @@ __global__ void k_features(const float *__restrict__ u, bf16_t *__restrict__ fea
         const float blend = (fx + fy + fz) * (1.0f / 3.0f);
         const bool shell = x == 0 || y == 0 || z == 0 || x == n - 1 || y == n - 1 || z == n - 1;
         const float bv = shell ? u[p] : 0.f;
+        // channels: blend, shell value, the three coordinates, the distance
+        // to the nearest face (scaled to [0, 1]), zeros
+        const float dist =
+            2.f * fminf(fminf(fminf(sx, 1.f - sx), fminf(sy, 1.f - sy)), fminf(sz, 1.f - sz));
         for (int c = 0; c < C; ++c) {
-            float v = c == 0 ? blend : (c == 1 ? bv : 0.f);
+            const float v = c == 0 ? blend
+                          : c == 1 ? bv
+                          : c == 2 ? sx
+                          : c == 3 ? sy
+                          : c == 4 ? sz
+                          : c == 5 ? dist
+                          : 0.f;
             feat[p * C + c] = f2bf(v);
         }
         if (base) base[p] = shell ? u[p] : blend;
