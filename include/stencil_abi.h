@@ -264,6 +264,12 @@ st_status st_device_sync(void);
 /* Streaming copy of n floats (n a multiple of 4, 16-byte aligned buffers):
  * the bandwidth probe that calibrates HBM counters against the sweeps. */
 st_status st_copy_f32(void *dst, const void *src, uint64_t n, void *stream);
+/* The 16-bit storage conversions the mixed form uses (dtype ST_BF16 or
+ * ST_F16), elementwise over n values: round to nearest even, and the exact
+ * widening back.  No reference counterpart (the reference has no 16-bit
+ * storage); for preparing and reading mixed-precision fields. */
+st_status stencil_round16(const float *d_in, uint16_t *d_out, uint64_t n, int dtype, void *stream);
+st_status stencil_widen16(const uint16_t *d_in, float *d_out, uint64_t n, int dtype, void *stream);
 
 #ifdef __cplusplus
 }

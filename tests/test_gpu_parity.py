@@ -432,3 +432,41 @@ def test_smem_register_rows_form_rejects_unsupported(gpu):
                                   iterations=2, **kw)
         with pytest.raises(st.StencilError):
             st.check_early_exit(p)
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "f16"])
+def test_round16_widen16_match_oracle(gpu, oracle, dtype):
+    """The 16-bit conversions of the mixed form against the oracle's integer
+    restatement: every 16-bit pattern widened, and FP32 values around every
+    16-bit rounding boundary (ties, subnormals, overflow) rounded."""
+    st = gpu
+    is_f16 = dtype == "f16"
+    dt = st.F16 if is_f16 else st.BF16
+    # widening: all 65536 patterns except NaNs (whose payloads may differ)
+    h = np.arange(65536, dtype=np.uint32).astype(np.uint16)
+    want_w = oracle.widen16(h, is_f16)
+    keep = ~np.isnan(want_w)
+    d_h = st.DeviceArray.from_numpy(h)
+    d_w = st.DeviceArray((65536,), np.float32)
+    st.widen16(d_h, d_w, 65536, dt)
+    got_w = d_w.numpy()
+    assert np.array_equal(got_w[keep].view(np.uint32), want_w[keep].view(np.uint32))
+    # rounding: each finite 16-bit value, its midpoints to both neighbours and
+    # one FP32 ulp either side of them, plus random floats over all exponents
+    w = want_w[keep & np.isfinite(want_w)].astype(np.float64)
+    w = np.unique(w)
+    mids = (w[1:] + w[:-1]) / 2
+    cand = np.concatenate([w, mids]).astype(np.float32)
+    bits = cand.view(np.uint32)
+    cand = np.concatenate([cand, (bits + 1).view(np.float32), (bits - 1).view(np.float32)])
+    rng = np.random.default_rng(7)
+    rnd = rng.integers(0, 2 ** 32, 1 << 20, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    x = np.concatenate([cand, rnd, np.float32([np.inf, -np.inf, 0.0, -0.0, 65504.0, 65520.0, 65519.99])])
+    x = x[~np.isnan(x)].astype(np.float32)
+    want_r = oracle.round16(x, is_f16)
+    d_x = st.DeviceArray.from_numpy(x)
+    d_r = st.DeviceArray((x.size,), np.uint16)
+    st.round16(d_x, d_r, x.size, dt)
+    got_r = d_r.numpy()
+    bad = np.flatnonzero(got_r != want_r)
+    assert bad.size == 0, f"{bad.size} of {x.size} differ, e.g. {x[bad[:4]]} -> {got_r[bad[:4]]} vs {want_r[bad[:4]]}"

@@ -175,6 +175,8 @@ def lib():
         L.st_stream_sync.argtypes = [P]
         L.st_device_sync.argtypes = []
         L.st_copy_f32.argtypes = [P, P, U64, P]
+        L.stencil_round16.argtypes = [P, P, U64, I, P]
+        L.stencil_widen16.argtypes = [P, P, U64, I, P]
         L.stencil_dispatch_batched.argtypes = [C.POINTER(Problem), I, C.POINTER(TileRule), I, I, P,
                                                C.POINTER(C.c_float)]
         for name in ("stencil_validate", "stencil_launch_config", "stencil_sweep",
@@ -182,7 +184,7 @@ def lib():
                      "stencil_dispatch_mixed", "st_device_count", "st_device_name", "st_malloc",
                      "st_free", "st_memcpy_htod", "st_memcpy_dtoh", "st_memcpy_dtod",
                      "st_memset", "st_stream_create", "st_stream_destroy", "st_stream_sync",
-                     "st_device_sync", "st_copy_f32",
+                     "st_device_sync", "st_copy_f32", "stencil_round16", "stencil_widen16",
                      "stencil_dispatch_batched"):
             getattr(L, name).restype = I
         _lib = L
@@ -213,6 +215,16 @@ def synchronize():
 def copy_f32(dst, src, n: int, stream=None):
     """Streaming-copy probe kernel (n floats)."""
     _check(lib().st_copy_f32(_ptr(dst), _ptr(src), n, _sh(stream)))
+
+
+def round16(d_in, d_out, n: int, dtype: int, stream=None):
+    """d_out (uint16) <- round-to-nearest-even of n floats to BF16 or F16."""
+    _check(lib().stencil_round16(_ptr(d_in), _ptr(d_out), n, dtype, _sh(stream)))
+
+
+def widen16(d_in, d_out, n: int, dtype: int, stream=None):
+    """d_out (float32) <- exact widening of n BF16 or F16 values."""
+    _check(lib().stencil_widen16(_ptr(d_in), _ptr(d_out), n, dtype, _sh(stream)))
 
 
 class Stream:

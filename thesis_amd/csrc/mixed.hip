@@ -190,3 +190,54 @@ st_status run_mixed(const Cfg &c, void *x, void *e1, void *e2, void *d, void *u,
 }
 
 }  // namespace st
+
+namespace st {
+
+template <bool F16>
+__global__ void k_round16(const float *__restrict__ in, uint16_t *__restrict__ out, long long n)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = rnd16<F16>(in[i]);
+}
+
+template <bool F16>
+__global__ void k_widen16(const uint16_t *__restrict__ in, float *__restrict__ out, long long n)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = wid16<F16>(in[i]);
+}
+
+static dim3 conv_blocks(uint64_t n)
+{
+    uint64_t b = (n + 255) / 256;
+    if (b > 65536) b = 65536;
+    return dim3((unsigned)(b < 1 ? 1 : b));
+}
+
+}  // namespace st
+
+extern "C" st_status stencil_round16(const float *d_in, uint16_t *d_out, uint64_t n, int dtype, void *stream)
+{
+    using namespace st;
+    if (dtype != ST_BF16 && dtype != ST_F16) return fail(ST_EINVAL, "round16: dtype must be BF16 or F16");
+    if (!d_in || !d_out) return fail(ST_EINVAL, "round16: null buffer");
+    if (n == 0) return ST_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (dtype == ST_F16) hipLaunchKernelGGL(k_round16<true>, conv_blocks(n), dim3(256), 0, s, d_in, d_out, (long long)n);
+    else hipLaunchKernelGGL(k_round16<false>, conv_blocks(n), dim3(256), 0, s, d_in, d_out, (long long)n);
+    return hip_check(hipGetLastError(), "round16 launch");
+}
+
+extern "C" st_status stencil_widen16(const uint16_t *d_in, float *d_out, uint64_t n, int dtype, void *stream)
+{
+    using namespace st;
+    if (dtype != ST_BF16 && dtype != ST_F16) return fail(ST_EINVAL, "widen16: dtype must be BF16 or F16");
+    if (!d_in || !d_out) return fail(ST_EINVAL, "widen16: null buffer");
+    if (n == 0) return ST_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (dtype == ST_F16) hipLaunchKernelGGL(k_widen16<true>, conv_blocks(n), dim3(256), 0, s, d_in, d_out, (long long)n);
+    else hipLaunchKernelGGL(k_widen16<false>, conv_blocks(n), dim3(256), 0, s, d_in, d_out, (long long)n);
+    return hip_check(hipGetLastError(), "widen16 launch");
+}

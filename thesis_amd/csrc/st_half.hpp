@@ -1,7 +1,7 @@
 // 16-bit storage conversions shared by the mixed-precision kernels:
-// round-to-nearest-even FP32 -> BF16 / FP16 and exact widening back, in
-// integer arithmetic identical to the oracle's (or_round16 / or_widen16 in
-// oracle/stencil_oracle.c), so results are bit-exact.
+// round-to-nearest-even FP32 -> BF16 / FP16 and exact widening back, equal
+// bit for bit to the oracle's integer restatement (or_round16 / or_widen16
+// in oracle/stencil_oracle.c) on finite values.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,48 +20,14 @@ __device__ __forceinline__ uint16_t to_bf16(float f)
 
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-__device__ __forceinline__ uint16_t to_f16(float f)
-{
-    const uint32_t x = __float_as_uint(f);
-    const uint32_t sign = (x >> 16) & 0x8000u;
-    const uint32_t ax = x & 0x7fffffffu;
-    if (ax > 0x7f800000u) return (uint16_t)(sign | 0x7e00u);
-    if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);
-    if (ax < 0x38800000u) {
-        if (ax < 0x33000000u) return (uint16_t)sign;
-        const uint32_t e = ax >> 23;
-        const uint32_t m = (ax & 0x7fffffu) | 0x800000u;
-        const uint32_t shift = 126u - e;
-        uint32_t q = m >> shift;
-        const uint32_t rem = m & ((1u << shift) - 1u);
-        const uint32_t half = 1u << (shift - 1u);
-        if (rem > half || (rem == half && (q & 1u))) q++;
-        return (uint16_t)(sign | q);
-    }
-    const uint32_t e = (ax >> 23) - 112u;
-    const uint32_t m = ax & 0x7fffffu;
-    uint32_t q = (e << 10) | (m >> 13);
-    const uint32_t rem = m & 0x1fffu;
-    if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) q++;
-    return (uint16_t)(sign | q);
-}
+// FP16: the hardware conversions.  v_cvt_f16_f32 rounds to nearest even
+// with FP16 subnormals kept (the default float mode of HIP kernels), which is
+// the integer restatement's rounding for every finite input and for
+// infinities; v_cvt_f32_f16 widens exactly.  (NaN payloads may differ from
+// the restatement, which returns a canonical quiet NaN.)
+__device__ __forceinline__ uint16_t to_f16(float f) { return __builtin_bit_cast(uint16_t, (_Float16)f); }
 
-__device__ __forceinline__ float from_f16(uint16_t h)
-{
-    const uint32_t sign = ((uint32_t)h & 0x8000u) << 16;
-    const uint32_t e = (h >> 10) & 0x1fu;
-    const uint32_t m = h & 0x3ffu;
-    uint32_t x;
-    if (e == 0) {
-        if (m == 0) x = sign;
-        else x = __float_as_uint((float)m * (1.0f / 16777216.0f)) | sign;
-    } else if (e == 31) {
-        x = sign | 0x7f800000u | (m << 13);
-    } else {
-        x = sign | ((e + 112u) << 23) | (m << 13);
-    }
-    return __uint_as_float(x);
-}
+__device__ __forceinline__ float from_f16(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
 
 template <bool F16> __device__ __forceinline__ uint16_t rnd16(float f) { return F16 ? to_f16(f) : to_bf16(f); }
 template <bool F16> __device__ __forceinline__ float wid16(uint16_t h) { return F16 ? from_f16(h) : from_bf16(h); }
