@@ -101,8 +101,14 @@ def test_launch_config_shapes(st):
                                                 block_x=32, block_y=8, block_z=4, coarsen_x=4))
     assert sm["lds_bytes"] == 32 * 4 * 8 * 4 * 4 and sm["grid_x"] == 4
     reg = st.launch_config(st.StencilParams.make(dim=512, radius=1, iterations=100,
-                                                 variant="smem_register", heuristic=1))
+                                                 variant="smem_register", heuristic=1, temporal_form=1))
     assert reg["block_x"] % 64 == 0 and reg["lds_bytes"] == 0 and reg["zchunk"] >= 16
+    # the library's SMEM_REGISTER choice: row streaming, whole rows, two LDS
+    # planes of (rows + 2R) rows of 8 cells per lane
+    rows = st.launch_config(st.StencilParams.make(dim=512, radius=1, iterations=100,
+                                                  variant="smem_register", heuristic=1))
+    assert rows["block_x"] == 512 and rows["coarsen_x"] == 8
+    assert rows["lds_bytes"] == 2 * (rows["block_y"] + 2) * 512 * 4
     coop = st.launch_config(st.StencilParams.make(dim=512, radius=1, iterations=100, variant="coop",
                                                   heuristic=1, op=st.OP_JACOBI))
     assert coop["temporal_steps"] >= 1 and coop["lds_bytes"] <= 160 * 1024

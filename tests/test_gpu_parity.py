@@ -394,7 +394,9 @@ ROWS_CASES = [
     ((12, 11, 13), "f32", (1, 3), {}),
     ((128, 50, 45), "f32", (1, 4), {"block_y": 12, "block_z": 11, "heuristic": 0}),
     ((64, 33, 29), "f64", (1, 2, 3, 4), {}),
-    ((300, 19, 18), "f64", (1, 2), {}),
+    ((300, 19, 18), "f64", (1, 2, 3), {}),           # 2 waves per row at R=2,3
+    ((1000, 13, 9), "f32", (1,), {}),                # 4 waves per row
+    ((640, 17, 11), "f32", (2,), {}),                # 3 waves per row, a short last part
 ]
 
 
@@ -426,8 +428,8 @@ def test_smem_register_rows_form(gpu, oracle, shape, dt, radii, knobs, op):
 
 def test_smem_register_rows_form_rejects_unsupported(gpu):
     st = gpu
-    for kw in (dict(nx=62, ny=40, nz=40), dict(nx=520, ny=40, nz=40), dict(dim=40, radius=5),
-               dict(nx=384, ny=20, nz=20, radius=3, dtype=st.F64), dict(nx=64, ny=40, nz=1, dimensions=2)):
+    for kw in (dict(nx=62, ny=40, nz=40), dict(nx=2052, ny=40, nz=40, radius=4), dict(dim=40, radius=5),
+               dict(nx=384, ny=20, nz=20, radius=4, dtype=st.F64), dict(nx=64, ny=40, nz=1, dimensions=2)):
         p = st.StencilParams.make(variant="smem_register", temporal_form=2, heuristic=1, op=1,
                                   iterations=2, **kw)
         with pytest.raises(st.StencilError):

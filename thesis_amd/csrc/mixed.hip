@@ -146,19 +146,14 @@ static st_status mixed_impl(const Cfg &c, float *x, uint16_t *e1, uint16_t *e2, 
     hipLaunchKernelGGL(k_fill16, blocks(n), dim3(256), 0, s, d, n);
     // 3D grids the row-streaming kernels take (FP32 arithmetic on 16-bit
     // loads, same operation order); the rest use the per-cell kernels
-    Cfg rc = c;
-    rc.dtype = ST_F32;
-    rc.op = OP_JACOBI;
-    rc.variant = ST_V_SMEM_REGISTER;
-    rc.heuristic = 1;
-    const bool rows = c.dims == 3 && rows_mixed_eligible(c) && resolve_rows(rc) == ST_OK;
+    const bool rows = c.dims == 3 && rows_mixed_eligible(c);
     st_status st = ST_OK;
     auto defect = [&]() {
-        if (rows) st = launch_rows_mixed(rc, 1, F16, x, d, nullptr, s);
+        if (rows) st = launch_rows_mixed(c, 1, F16, x, d, nullptr, s);
         else hipLaunchKernelGGL(k_mx_defect<F16>, blocks(ni), dim3(256), 0, s, x, d, a);
     };
     auto sweep = [&](const uint16_t *ein, uint16_t *eout) {
-        if (rows) st = launch_rows_mixed(rc, 0, F16, ein, eout, d, s);
+        if (rows) st = launch_rows_mixed(c, 0, F16, ein, eout, d, s);
         else hipLaunchKernelGGL(k_mx_sweep<F16>, blocks(ni), dim3(256), 0, s, ein, d, eout, a);
     };
     defect();

@@ -62,10 +62,11 @@ st_status run_tblock(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2
 bool rows_eligible(const Cfg &c);
 st_status resolve_rows(Cfg &c);
 st_status launch_rows(const Cfg &c, const void *in, void *out, hipStream_t s);
-// 16-bit storage modes: `rc` resolved as an FP32 JACOBI rows sweep; defect:
-// d = round16(w*sum(x) - x), else e' = round16(widen(d) + w*sum(widen(e)))
+// 16-bit storage modes on the geometry of `c` (FP32 arithmetic, JACOBI
+// weight c.weight): defect d = round16(w*sum(x) - x), else the correction
+// e' = round16(widen(d) + w*sum(widen(e)))
 bool rows_mixed_eligible(const Cfg &c);
-st_status launch_rows_mixed(const Cfg &rc, int defect, bool f16, const void *in, void *out, const uint16_t *d,
+st_status launch_rows_mixed(const Cfg &c, int defect, bool f16, const void *in, void *out, const uint16_t *d,
                             hipStream_t s);
 
 // Streaming temporal blocking, whole rows per wave (sweep_stream.hip).

@@ -39,7 +39,8 @@ namespace st {
 struct RowsArgs {
     int nx, ny, nz;
     long long sz;   // nx*ny
-    int wy;         // core rows per frame (= waves per workgroup)
+    int wx;         // waves per row (4 cells per lane when > 1)
+    int wy;         // core rows per frame
     int tiles_y, zchunk, ntiles;
     int nt;         // non-temporal stores
 };
@@ -109,41 +110,51 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE>())) void k_rows(
     QV *lds = reinterpret_cast<QV *>(lds_raw);
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wy = a.wy;
-    const int lrows = wy + 2 * R;  // LDS rows per buffer: R halo, wy core, R halo
+    const int wx = a.wx, wy = a.wy;
+    const int wr = w / wx, wxp = w - wr * wx;  // row of the frame, part of the row
+    const int QW = wx * H;                     // quads per LDS row
+    const int lrows = wy + 2 * R;              // LDS rows per buffer: R halo, wy core, R halo
     const bool first = lane == 0, last = lane == 63;
 
+    // own quads: columns 256*(wxp*H + h) + 4*lane
     int xo[H];
     bool act[H], intq[H];
 #pragma unroll
     for (int h = 0; h < H; ++h) {
-        const int x = 256 * h + 4 * lane;
+        const int x = 256 * (wxp * H + h) + 4 * lane;
         act[h] = x < a.nx;
         xo[h] = act[h] ? x : a.nx - 4;  // quads past the row reload its last cells
         intq[h] = act[h] && x >= R && x + 4 <= a.nx - R;
     }
+    // halo strip of this wave (waves 0..2R*wx-1): row hr of the 2R halo rows,
+    // part hp of it
+    const bool has_halo = w < 2 * R * wx;
+    const int hr = w / wx, hp = w - hr * wx;
+    int hxo[H];
+#pragma unroll
+    for (int h = 0; h < H; ++h) hxo[h] = min(256 * (hp * H + h) + 4 * lane, a.nx - 4);
     const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
 
     for (int tile = (int)bid; tile < a.ntiles; tile += (int)gridDim.x) {
         const int tyi = tile % a.tiles_y, tzi = tile / a.tiles_y;
         const int fy0 = tyi * wy;
-        const int gy = fy0 + w;  // own (core) row
+        const int gy = fy0 + wr;  // own (core) row
         const int own_off = min(gy, a.ny - 1) * a.nx;
         const bool store_row = gy >= R && gy < a.ny - R;
-        // halo row of this wave (waves 0..2R-1): frame rows -R..-1 and wy..wy+R-1
-        const bool has_halo = w < 2 * R;
-        const int hrow = w < R ? w : wy + w;                        // LDS row
-        const int hgy = w < R ? fy0 - R + w : fy0 + wy + (w - R);   // grid row
+        // halo rows: frame rows -R..-1 (LDS rows 0..R-1), wy..wy+R-1 (LDS
+        // rows R+wy..)
+        const int hrow = hr < R ? hr : wy + hr;
+        const int hgy = hr < R ? fy0 - R + hr : fy0 + wy + (hr - R);
         const int halo_off = min(max(hgy, 0), a.ny - 1) * a.nx;
         const int z0 = tzi * a.zchunk, z1 = min(z0 + a.zchunk, a.nz);
 
-        auto load_row = [&](int k, int off, T (&d)[CX]) {
+        auto load_row = [&](int k, int off, const int (&cols)[H], T (&d)[CX]) {
             k = min(max(k, 0), a.nz - 1);
             if constexpr (MODE == RM_CORR) {
                 const uint16_t *p = static_cast<const uint16_t *>(in_) + (long long)k * a.sz + off;
 #pragma unroll
                 for (int h = 0; h < H; ++h) {
-                    const U4 q = *reinterpret_cast<const U4 *>(p + xo[h]);
+                    const U4 q = *reinterpret_cast<const U4 *>(p + cols[h]);
 #pragma unroll
                     for (int j = 0; j < 4; ++j) d[4 * h + j] = wid16<F16>(q[j]);
                 }
@@ -151,7 +162,7 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE>())) void k_rows(
                 const T *p = static_cast<const T *>(in_) + (long long)k * a.sz + off;
 #pragma unroll
                 for (int h = 0; h < H; ++h) {
-                    const QV q = *reinterpret_cast<const QV *>(p + xo[h]);
+                    const QV q = *reinterpret_cast<const QV *>(p + cols[h]);
 #pragma unroll
                     for (int j = 0; j < 4; ++j) d[4 * h + j] = q[j];
                 }
@@ -165,20 +176,20 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE>())) void k_rows(
 #pragma unroll
             for (int c = 0; c < CX; ++c) ring[i][c] = part[i][c] = (T)0;
         const int kin_lo = z0 - R, kin_hi = z1 + R;
-        load_row(kin_lo, own_off, pf);
-        if (has_halo) load_row(kin_lo, halo_off, ph);
+        load_row(kin_lo, own_off, xo, pf);
+        if (has_halo) load_row(kin_lo, halo_off, hxo, ph);
 
         for (int m = kin_lo; m < kin_hi; ++m) {
             T nw[CX];
 #pragma unroll
             for (int c = 0; c < CX; ++c) nw[c] = pf[c];
-            QV *buf = lds + (size_t)((m & 1) * lrows) * H * 64;
+            QV *buf = lds + (size_t)((m & 1) * lrows) * QW * 64;
 #pragma unroll
             for (int h = 0; h < H; ++h) {
                 QV q;
 #pragma unroll
                 for (int j = 0; j < 4; ++j) q[j] = nw[4 * h + j];
-                buf[((R + w) * H + h) * 64 + lane] = q;
+                buf[((R + wr) * QW + wxp * H + h) * 64 + lane] = q;
             }
             if (has_halo) {
 #pragma unroll
@@ -186,12 +197,12 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE>())) void k_rows(
                     QV q;
 #pragma unroll
                     for (int j = 0; j < 4; ++j) q[j] = ph[4 * h + j];
-                    buf[(hrow * H + h) * 64 + lane] = q;
+                    buf[(hrow * QW + hp * H + h) * 64 + lane] = q;
                 }
             }
             if (m + 1 < kin_hi) {
-                load_row(m + 1, own_off, pf);
-                if (has_halo) load_row(m + 1, halo_off, ph);
+                load_row(m + 1, own_off, xo, pf);
+                if (has_halo) load_row(m + 1, halo_off, hxo, ph);
             }
             // correction sweep: the defect at the cell this step completes
             // (plane m-R), loaded now and used after the barrier
@@ -216,6 +227,26 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE>())) void k_rows(
                     pq[h][j] = dpp_prev(nw[4 * h + j]);
                     nq[h][j] = dpp_next(nw[4 * h + j]);
                 }
+            // lane 0 of the first quad and lane 63 of the last take the
+            // neighbouring wave's edge quad when the row spans several waves
+            // (LDS broadcast reads), else the rotate's wrap (unused values
+            // next to the shell)
+            T lq[4], rq[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                lq[j] = pq[0][j];
+                rq[j] = nq[H - 1][j];
+            }
+            if (wx > 1) {
+                const QV le = buf[((R + wr) * QW + max(wxp * H - 1, 0)) * 64 + 63];
+                const QV re = buf[((R + wr) * QW + min(wxp * H + H, QW - 1)) * 64];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const T l0 = le[j], r0 = re[j];
+                    lq[j] = wxp > 0 ? l0 : lq[j];
+                    rq[j] = wxp < wx - 1 ? r0 : rq[j];
+                }
+            }
             T s[CX];
 #pragma unroll
             for (int h = 0; h < H; ++h) {
@@ -229,7 +260,7 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE>())) void k_rows(
                             v = nw[4 * h + j - d];
                         } else {
                             const T mine = pq[h][4 + j - d];
-                            const T other = h > 0 ? pq[h - 1][4 + j - d] : mine;
+                            const T other = h > 0 ? pq[h - 1][4 + j - d] : lq[4 + j - d];
                             v = first ? other : mine;
                         }
                         u += v;
@@ -241,7 +272,7 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE>())) void k_rows(
                             v = nw[4 * h + j + d];
                         } else {
                             const T mine = nq[h][j + d - 4];
-                            const T other = h < H - 1 ? nq[h + 1][j + d - 4] : mine;
+                            const T other = h < H - 1 ? nq[h + 1][j + d - 4] : rq[j + d - 4];
                             v = last ? other : mine;
                         }
                         u += v;
@@ -249,12 +280,12 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE>())) void k_rows(
                     s[4 * h + j] = u;
                 }
             }
-            // y terms from the LDS plane: rows R+w-d (d = R..1), then R+w+d
+            // y terms from the LDS plane: rows R+wr-d (d = R..1), then R+wr+d
 #pragma unroll
             for (int d = R; d >= 1; --d) {
 #pragma unroll
                 for (int h = 0; h < H; ++h) {
-                    const QV q = buf[((R + w - d) * H + h) * 64 + lane];
+                    const QV q = buf[((R + wr - d) * QW + wxp * H + h) * 64 + lane];
 #pragma unroll
                     for (int j = 0; j < 4; ++j) s[4 * h + j] += q[j];
                 }
@@ -263,7 +294,7 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE>())) void k_rows(
             for (int d = 1; d <= R; ++d) {
 #pragma unroll
                 for (int h = 0; h < H; ++h) {
-                    const QV q = buf[((R + w + d) * H + h) * 64 + lane];
+                    const QV q = buf[((R + wr + d) * QW + wxp * H + h) * 64 + lane];
 #pragma unroll
                     for (int j = 0; j < 4; ++j) s[4 * h + j] += q[j];
                 }
@@ -345,13 +376,6 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE>())) void k_rows(
 
 // ---------------------------------------------------------------- host side
 
-static int rows_cx(const Cfg &c)
-{
-    if (c.nx % 4 || c.nx > 512) return 0;
-    return c.nx <= 256 ? 4 : 8;
-}
-
-
 template <int R, int CX, typename T, int MODE, bool F16>
 static const void *rows_fn_t(int op)
 {
@@ -389,9 +413,10 @@ static const void *rows_fn_d(int op, int r, int cx, int *threads)
 
 // mode RM_SWEEP: c.dtype F32/F64; RM_CORR / RM_DEFECT: FP32 arithmetic on
 // 16-bit storage (f16 selects FP16, else BF16)
-static const void *rows_fn_mode(const Cfg &c, int mode, bool f16, int *threads)
+static const void *rows_fn_mode(const Cfg &c, int cx, int mode, bool f16, int *threads)
 {
-    const int cx = rows_cx(c);
+    *threads = 0;
+    if (c.r < 1 || c.r > 4) return nullptr;
     if (mode == RM_CORR)
         return f16 ? rows_fn_d<float, RM_CORR, true>(c.op, c.r, cx, threads)
                    : rows_fn_d<float, RM_CORR, false>(c.op, c.r, cx, threads);
@@ -402,36 +427,65 @@ static const void *rows_fn_mode(const Cfg &c, int mode, bool f16, int *threads)
     return rows_fn_d<float, RM_SWEEP, false>(c.op, c.r, cx, threads);
 }
 
-static const void *rows_fn(const Cfg &c, int *threads) { return rows_fn_mode(c, RM_SWEEP, false, threads); }
+// Lane layout: 4 cells per lane and one wave per row up to 256 columns; up
+// to 512, 8 cells per lane for sweeps where that kernel is built (measured
+// on 384^3 and 512^3: equal or faster at every radius, up to 1.19x at R=3)
+// and ceil(nx/256) waves of 4 cells for the 16-bit modes (1.12x / 1.17x
+// for BF16 / FP16 at 384^3 R=4); beyond 512, ceil(nx/256) waves of 4 cells.
+// Returns false when no built kernel fits (a frame must hold the 2R halo
+// strips: rows >= 2R).
+static bool rows_layout(const Cfg &c, int mode, bool f16, int *cx, int *wx, int *threads,
+                        const void **fn)
+{
+    if (c.nx % 4 || c.nx < 8 || c.r < 1 || c.r > 4) return false;
+    int t8 = 0;
+    const void *f8 = c.nx <= 512 ? rows_fn_mode(c, 8, mode, f16, &t8) : nullptr;
+    // coarsen_x 4 or 8 with HEURISTIC=0 picks the layout explicitly
+    const int want = (!c.heuristic && (c.cx == 4 || c.cx == 8)) ? c.cx : 0;
+    if (c.nx <= 256 && want != 8) {
+        *cx = 4;
+        *wx = 1;
+    } else if (f8 && (want == 8 || (want == 0 && mode == RM_SWEEP))) {
+        *cx = 8;
+        *wx = 1;
+    } else {
+        if (want == 8) return false;
+        *cx = 4;
+        *wx = (c.nx + 255) / 256;
+    }
+    *fn = *cx == 8 ? f8 : rows_fn_mode(c, 4, mode, f16, threads);
+    if (*cx == 8) *threads = t8;
+    if (!*fn) return false;
+    return (*threads / 64) / *wx >= 2 * c.r;
+}
 
 bool rows_eligible(const Cfg &c)
 {
-    if (!((c.dtype == ST_F32 || c.dtype == ST_F64) && c.dims == 3 && c.r >= 1 && c.r <= 4 && c.nx >= 8 &&
-          rows_cx(c) != 0))
-        return false;
-    int threads = 0;
-    return rows_fn(c, &threads) != nullptr;
+    if (!((c.dtype == ST_F32 || c.dtype == ST_F64) && c.dims == 3)) return false;
+    int cx, wx, threads;
+    const void *fn;
+    return rows_layout(c, RM_SWEEP, false, &cx, &wx, &threads, &fn);
 }
 
-st_status resolve_rows(Cfg &c)
+static st_status rows_resolve_mode(Cfg &c, int mode, bool f16)
 {
-    if (!rows_eligible(c))
-        return fail(ST_EUNSUPPORTED, "row-streaming form needs a 3D F32/F64 grid, radius 1..4, nx a "
-                                     "multiple of 4 up to 512 (FP64 with nx > 256: radius 1..2)");
-    const int cx = rows_cx(c);
-    int threads = 0;
-    (void)rows_fn(c, &threads);
-    int wy = threads / 64;
+    int cx = 0, wx = 1, threads = 0;
+    const void *fn = nullptr;
+    if (c.dims != 3 || !rows_layout(c, mode, f16, &cx, &wx, &threads, &fn))
+        return fail(ST_EUNSUPPORTED, "row-streaming form needs a 3D F32/F64 grid, radius 1..4 and nx a "
+                                     "multiple of 4 whose rows fit a workgroup (at least 2R rows per frame)");
+    const int rows_max = threads / 64 / wx;
+    int wy = rows_max;
     if (!c.heuristic && c.by > 0) {
-        // block_y = rows per frame (waves per workgroup)
-        if (c.by < 2 * c.r || c.by > wy)
-            return fail(ST_EINVAL, "row-streaming form: block_y (rows per frame) must be %d..%d", 2 * c.r, wy);
+        // block_y = rows per frame
+        if (c.by < 2 * c.r || c.by > rows_max)
+            return fail(ST_EINVAL, "row-streaming form: block_y (rows per frame) must be %d..%d", 2 * c.r, rows_max);
         wy = c.by;
     } else if (wy > c.ny) {
         wy = c.ny < 2 * c.r ? 2 * c.r : c.ny;
     }
-    const int es = c.dtype == ST_F64 ? 8 : 4;
-    c.lds = 2 * (wy + 2 * c.r) * (cx / 4) * 64 * 4 * es;
+    const int es = (c.dtype == ST_F64 && mode == RM_SWEEP) ? 8 : 4;
+    c.lds = 2 * (wy + 2 * c.r) * wx * (cx / 4) * 64 * 4 * es;
     if (c.lds > kMaxLds) return fail(ST_ELDS, "row-streaming form needs %d bytes of LDS", c.lds);
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -450,6 +504,7 @@ st_status resolve_rows(Cfg &c)
     c.kind = 4;
     c.cx = cx;
     c.cy = 1;
+    c.wx = wx;
     c.wy = wy;
     c.bx = c.nx;
     c.by = wy;
@@ -460,12 +515,15 @@ st_status resolve_rows(Cfg &c)
     return ST_OK;
 }
 
+st_status resolve_rows(Cfg &c) { return rows_resolve_mode(c, RM_SWEEP, false); }
+
 static st_status rows_launch(const Cfg &c, const void *fn, const void *in, void *out, const uint16_t *aux,
                              hipStream_t s)
 {
     RowsArgs a;
     a.nx = c.nx; a.ny = c.ny; a.nz = c.nz;
     a.sz = (long long)c.nx * c.ny;
+    a.wx = c.wx;
     a.wy = c.wy;
     a.tiles_y = (c.ny + c.wy - 1) / c.wy;
     a.zchunk = c.zchunk;
@@ -473,7 +531,7 @@ static st_status rows_launch(const Cfg &c, const void *fn, const void *in, void 
     a.nt = 1;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds);
     if (e != hipSuccess) return hip_check(e, "hipFuncSetAttribute(rows LDS)");
-    const int nthreads = 64 * c.wy;
+    const int nthreads = 64 * c.wx * c.wy;
     if (c.dtype == ST_F64) {
         double kd = c.op == OP_THESIS ? (double)(2 * c.dims * c.r) : c.weight;
         void *args[] = {&in, &out, &aux, &a, &kd};
@@ -490,29 +548,37 @@ static st_status rows_launch(const Cfg &c, const void *fn, const void *in, void 
 st_status launch_rows(const Cfg &c, const void *in, void *out, hipStream_t s)
 {
     int threads = 0;
-    const void *fn = rows_fn(c, &threads);
-    if (!fn) return fail(ST_EUNSUPPORTED, "row-streaming kernel missing");
+    const void *fn = rows_fn_mode(c, c.cx, RM_SWEEP, false, &threads);
+    if (!fn || 64 * c.wx * c.wy > threads) return fail(ST_EUNSUPPORTED, "row-streaming kernel missing");
     return rows_launch(c, fn, in, out, nullptr, s);
 }
 
+// The 16-bit modes resolve their own layout (their register need differs
+// from the sweep's by the defect row).
 bool rows_mixed_eligible(const Cfg &c)
 {
     Cfg f = c;
     f.dtype = ST_F32;
     f.op = OP_JACOBI;
-    if (!rows_eligible(f)) return false;
-    int threads = 0;
-    return rows_fn_mode(f, RM_CORR, c.dtype == ST_F16, &threads) &&
-           rows_fn_mode(f, RM_DEFECT, c.dtype == ST_F16, &threads);
+    f.heuristic = 1;
+    Cfg g = f;
+    return f.dims == 3 && rows_resolve_mode(f, RM_CORR, c.dtype == ST_F16) == ST_OK &&
+           rows_resolve_mode(g, RM_DEFECT, c.dtype == ST_F16) == ST_OK;
 }
 
-st_status launch_rows_mixed(const Cfg &rc, int defect, bool f16, const void *in, void *out, const uint16_t *d,
+st_status launch_rows_mixed(const Cfg &c, int defect, bool f16, const void *in, void *out, const uint16_t *d,
                             hipStream_t s)
 {
+    Cfg rc = c;
+    rc.dtype = ST_F32;
+    rc.op = OP_JACOBI;
+    rc.heuristic = 1;
+    const int mode = defect ? RM_DEFECT : RM_CORR;
+    st_status st = rows_resolve_mode(rc, mode, f16);
+    if (st != ST_OK) return st;
     int threads = 0;
-    const void *fn = rows_fn_mode(rc, defect ? RM_DEFECT : RM_CORR, f16, &threads);
-    if (!fn) return fail(ST_EUNSUPPORTED, "row-streaming mixed kernel missing");
-    if (64 * rc.wy > threads) return fail(ST_EINVAL, "row-streaming mixed kernel: frame too tall");
+    const void *fn = rows_fn_mode(rc, rc.cx, mode, f16, &threads);
+    if (!fn || 64 * rc.wx * rc.wy > threads) return fail(ST_EUNSUPPORTED, "row-streaming mixed kernel missing");
     return rows_launch(rc, fn, in, out, d, s);
 }
 

@@ -381,11 +381,28 @@ def jacobi_to_tolerance(u1: DeviceArray, u2: DeviceArray, n: int, r0: float, tol
     return sweeps, rel
 
 
+def coarse_refine(u: DeviceArray, scratch: DeviceArray, m: int, sweeps: int, stream):
+    """`sweeps` (even) JACOBI sweeps of the m^3 field u in place (scratch is
+    the second buffer; both hold u's shell)."""
+    if sweeps <= 0:
+        return
+    assert sweeps % 2 == 0
+    scratch.copy_from(u, stream)
+    p = StencilParams.make(dim=m, radius=1, iterations=sweeps, op=OP_JACOBI, variant="auto", heuristic=1)
+    in_u2, _ = dispatch_kernels(p, u, scratch, stream)
+    assert not in_u2
+
+
 def compare_warm_start(state: dict, seed: int, n: int = 128, m: int = 64, tol: float = 1e-6,
-                       check_every: int = 200, max_sweeps: int = 200000, stream=None):
+                       check_every: int = 200, max_sweeps: int = 200000, stream=None,
+                       coarse_sweeps: int = 0):
     """Plain zero-start JACOBI versus the learned warm start on one 128^3
     problem with Fourier-mode faces (seed).  Relative residuals use the
-    zero-start residual norm as the common denominator.  Returns a dict."""
+    zero-start residual norm as the common denominator.  With coarse_sweeps
+    > 0, two more starts are timed: the network's m^3 prediction refined by
+    that many m^3 JACOBI sweeps before the upsampling ("learned_coarse"),
+    and the same refinement of the network's input guess alone
+    ("blend_coarse", the control without the network).  Returns a dict."""
     s = stream or Stream()
     modes = DeviceArray.from_numpy(face_modes(1, seed), s)
     u1 = DeviceArray((n, n, n), np.float32)
@@ -435,6 +452,34 @@ def compare_warm_start(state: dict, seed: int, n: int = 128, m: int = 64, tol: f
     out["blend"] = {"sweeps": sw, "seconds": time.perf_counter() - t0, "rel_residual": rel,
                     "start_rel_residual": rel_b}
     out["time_ratio_learned_vs_plain"] = out["learned"]["seconds"] / out["plain"]["seconds"]
+    if coarse_sweeps > 0:
+        cm = DeviceArray((m, m, m), np.float32)
+        cs = DeviceArray((m, m, m), np.float32)
+        fm = DeviceArray((m ** 3, FEAT_C), np.uint16)
+        for label in ("learned_coarse", "blend_coarse"):
+            u1.fill_zero(s)
+            nn.fourier_faces(u1, modes, 1, n, MODES, s)
+            s.sync()
+            t0 = time.perf_counter()
+            if label == "learned_coarse":
+                net.forward()
+                cm.copy_from(net.pred, s)
+            else:
+                cm.fill_zero(s)
+                nn.fourier_faces(cm, modes, 1, m, MODES, s)
+                nn.features(cm, fm, cs, 1, m, FEAT_C, s)
+                cm.copy_from(cs, s)
+            coarse_refine(cm, cs, m, coarse_sweeps, s)
+            nn.trilinear(cm, u1, m, n, s)
+            u2.copy_from(u1, s)
+            sw, rel = jacobi_to_tolerance(u1, u2, n, r0, tol, check_every, max_sweeps, s)
+            s.sync()
+            out[label] = {"sweeps": sw, "seconds": time.perf_counter() - t0, "rel_residual": rel,
+                          "coarse_sweeps": coarse_sweeps}
+        out["time_ratio_learned_coarse_vs_plain"] = out["learned_coarse"]["seconds"] / out["plain"]["seconds"]
+        out["time_ratio_blend_coarse_vs_plain"] = out["blend_coarse"]["seconds"] / out["plain"]["seconds"]
+        for t in (cm, cs, fm):
+            t.free()
     for t in (modes, u1, u2, feat, base):
         t.free()
     return out
@@ -467,6 +512,8 @@ def main(argv=None):
     ap.add_argument("--problems", type=int, default=3)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--check-every", type=int, default=200)
+    ap.add_argument("--coarse-sweeps", type=int, default=0,
+                    help="eval: also time the prediction refined by this many (even) coarse sweeps")
     ap.add_argument("--smooth-sweeps", type=int, default=64)
     ap.add_argument("--smooth-weight", type=float, default=100.0)
     a = ap.parse_args(argv)
@@ -493,16 +540,28 @@ def main(argv=None):
         results = []
         for i in range(a.problems):
             r = compare_warm_start(state, seed=a.seed + 1_000_000 + i, n=a.fine, m=a.n, tol=a.tol,
-                                   check_every=a.check_every, stream=s)
-            print(f"[warmstart] problem {i}: plain {r['plain']['sweeps']} sweeps "
-                  f"{r['plain']['seconds']:.3f} s, learned {r['learned']['sweeps']} sweeps "
-                  f"{r['learned']['seconds']:.3f} s (ratio {r['time_ratio_learned_vs_plain']:.3f}), "
-                  f"blend {r['blend']['sweeps']} sweeps", flush=True)
+                                   check_every=a.check_every, stream=s, coarse_sweeps=a.coarse_sweeps)
+            line = (f"[warmstart] problem {i}: plain {r['plain']['sweeps']} sweeps "
+                    f"{r['plain']['seconds']:.3f} s, learned {r['learned']['sweeps']} sweeps "
+                    f"{r['learned']['seconds']:.3f} s (ratio {r['time_ratio_learned_vs_plain']:.3f}), "
+                    f"blend {r['blend']['sweeps']} sweeps")
+            if a.coarse_sweeps > 0:
+                line += (f"; +{a.coarse_sweeps} coarse sweeps: learned {r['learned_coarse']['sweeps']} "
+                         f"sweeps (ratio {r['time_ratio_learned_coarse_vs_plain']:.3f}), blend "
+                         f"{r['blend_coarse']['sweeps']} sweeps (ratio "
+                         f"{r['time_ratio_blend_coarse_vs_plain']:.3f})")
+            print(line, flush=True)
             results.append(r)
         ratios = [r["time_ratio_learned_vs_plain"] for r in results]
-        print(json.dumps({"metric": "time-to-1e-6 residual, learned warm start vs plain zero start",
-                          "value": float(np.mean(ratios)), "unit": "ratio", "higher_is_better": False,
-                          "results": results}), flush=True)
+        doc = {"metric": "time-to-1e-6 residual, learned warm start vs plain zero start",
+               "value": float(np.mean(ratios)), "unit": "ratio", "higher_is_better": False,
+               "results": results}
+        if a.coarse_sweeps > 0:
+            doc["learned_coarse_mean_ratio"] = float(np.mean([r["time_ratio_learned_coarse_vs_plain"]
+                                                              for r in results]))
+            doc["blend_coarse_mean_ratio"] = float(np.mean([r["time_ratio_blend_coarse_vs_plain"]
+                                                            for r in results]))
+        print(json.dumps(doc), flush=True)
 
 
 if __name__ == "__main__":
