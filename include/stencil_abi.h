@@ -224,8 +224,11 @@ typedef struct st_problem {
 typedef struct st_tile_rule {
     int radius;
     long long max_cells;
-    int coarsen_y;        /* rows per wave: 1, 2 or 4 (radius 2: 1 or 2) */
-    int zchunk;           /* planes per wave tile (0 = whole z extent)    */
+    int coarsen_y;        /* rows per tile frame (0 = as many as fit): the
+                             plane-chunk form, used when every problem has
+                             radius 1..2 and (1+2R)*nx <= 4096; otherwise
+                             rows per wave of the march form (1, 2 or 4) */
+    int zchunk;           /* planes per tile (0 = whole z extent)         */
 } st_tile_rule;
 
 typedef enum st_batch_mode {

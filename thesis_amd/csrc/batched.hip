@@ -5,9 +5,11 @@
 // stencil_dispatch (ping-pong between its u1 and u2, result in u1 for an
 // even iteration count, bit-exact with the CPU oracle).
 //
-//   * all problems' work is one flat list of wave tiles (register-queue
-//     march tiles, 64 columns x TY rows x zchunk planes) ordered by problem;
-//     a wave finds its problem by binary search over the list offsets;
+//   * all problems' work is one flat list of tiles ordered by problem; a
+//     tile finds its problem by binary search over the list offsets.  The
+//     tiles are workgroup tiles of the plane-chunk form (below) when every
+//     problem fits it, else wave tiles of the register-queue march (64
+//     columns x TY rows x zchunk planes);
 //   * ST_BATCH_GRAPH captures one launch per sweep (the whole batch in each)
 //     into a hipGraph and replays it as a single graph launch;
 //   * ST_BATCH_PERSISTENT runs every sweep inside one cooperative kernel
@@ -99,6 +101,159 @@ __global__ __launch_bounds__(256) void k_batch_persistent(const BatchProblem *__
     }
 }
 
+// ---------------------------------------------------------------- plane-chunk form
+//
+// A workgroup of 256 threads owns a tile: frame rows [y0, y0+F) of one
+// problem over a z chunk.  A frame's rows of one plane, with R halo rows on
+// each side, are one contiguous run of memory, so every plane is fetched
+// with fully coalesced loads of consecutive floats (any nx), prefetched
+// into registers one plane ahead and written to a double-buffered LDS
+// plane (two planes in flight).  Each thread owns up to kBpCells cells of the frame (t, t+256, ...,
+// contiguous in memory as well); their x and y neighbours are read from the
+// LDS plane and z runs through a ring of R planes and R partial sums, in the
+// reference's order, as in sweep_rows.hip.
+constexpr int kBpThreads = 256;
+constexpr int kBpCells = 8;    // owned cells per thread: frame core <= 2048 cells
+constexpr int kBpLoads = 16;   // prefetched floats per thread: frame + halo <= 4096
+
+template <int OP, int R>
+__device__ __forceinline__ void bp_tile(const BatchProblem &q, int local, int sweep, float *lds)
+{
+    const int t = threadIdx.x;
+    const int nx = q.g.nx, ny = q.g.ny, nz = q.g.nz;
+    const long long sz = q.g.sz;
+    const int ty = local % q.tiles_y, tz = local / q.tiles_y;
+    const int y0 = ty * q.ty;                 // ty holds the frame height F
+    const int fr = min(q.ty, ny - y0);        // rows of this frame
+    const int ncore = fr * nx;
+    const int z0 = tz * q.zchunk, z1 = min(z0 + q.zchunk, nz);
+    const float *in = (sweep & 1) ? q.u2 : q.u1;
+    float *out = (sweep & 1) ? q.u1 : q.u2;
+    // contiguous load window of a plane: rows [rlo, rhi) of the domain
+    const int rlo = max(y0 - R, 0), rhi = min(y0 + fr + R, ny);
+    const int nload = (rhi - rlo) * nx;
+    const int ldoff = (rlo - (y0 - R)) * nx;  // LDS offset of row rlo
+    const int plane = (fr + 2 * R) * nx;      // LDS floats per buffer
+
+    // owned cells: interior mask (x, y, plus the z test per plane)
+    unsigned own = 0, intr = 0;
+#pragma unroll
+    for (int i = 0; i < kBpCells; ++i) {
+        const int c = t + i * kBpThreads;
+        if (c < ncore) {
+            own |= 1u << i;
+            const int r = c / nx, x = c - r * nx;
+            const int y = y0 + r;
+            if (x >= R && x < nx - R && y >= R && y < ny - R) intr |= 1u << i;
+        }
+    }
+
+    float ring[R][kBpCells], part[R][kBpCells], pf[kBpLoads], pf2[kBpLoads];
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+#pragma unroll
+        for (int i = 0; i < kBpCells; ++i) ring[j][i] = part[j][i] = 0.0f;
+
+    auto load_plane = [&](int k, float (&d)[kBpLoads]) {
+        k = min(max(k, 0), nz - 1);
+        const float *p = in + (long long)k * sz + (long long)rlo * nx;
+#pragma unroll
+        for (int i = 0; i < kBpLoads; ++i) {
+            const int j = t + i * kBpThreads;
+            if (j < nload) d[i] = p[j];
+        }
+    };
+    // two planes in flight: pf holds plane m when step m starts, pf2 plane
+    // m+1
+    const int kin_lo = z0 - R, kin_hi = z1 + R;
+    load_plane(kin_lo, pf);
+    if (kin_lo + 1 < kin_hi) load_plane(kin_lo + 1, pf2);
+    for (int m = kin_lo; m < kin_hi; ++m) {
+        float *buf = lds + (m & 1) * plane;
+#pragma unroll
+        for (int i = 0; i < kBpLoads; ++i) {
+            const int j = t + i * kBpThreads;
+            if (j < nload) buf[ldoff + j] = pf[i];
+        }
+#pragma unroll
+        for (int i = 0; i < kBpLoads; ++i) pf[i] = pf2[i];
+        if (m + 2 < kin_hi) load_plane(m + 2, pf2);
+        __syncthreads();
+        const int k = m - R;  // plane completed this step
+        const bool kin = k >= z0 && k < z1 && k >= R && k < nz - R;
+        float *orow = out + (long long)k * sz + (long long)y0 * nx;
+#pragma unroll
+        for (int i = 0; i < kBpCells; ++i) {
+            if (!((own >> i) & 1u)) continue;
+            const int b = R * nx + t + i * kBpThreads;  // LDS index of the cell
+            float s = 0.0f;
+#pragma unroll
+            for (int d = R; d >= 1; --d) s += buf[b - d];
+#pragma unroll
+            for (int d = 1; d <= R; ++d) s += buf[b + d];
+#pragma unroll
+            for (int d = R; d >= 1; --d) s += buf[b - d * nx];
+#pragma unroll
+            for (int d = 1; d <= R; ++d) s += buf[b + d * nx];
+#pragma unroll
+            for (int j = 0; j < R; ++j) s += ring[j][i];
+            const float cen = buf[b];
+#pragma unroll
+            for (int j = 0; j < R; ++j) part[j][i] += cen;
+            if (kin && ((intr >> i) & 1u))
+                orow[t + i * kBpThreads] = finish<OP>(part[0][i], ring[0][i], q.kc);
+#pragma unroll
+            for (int j = 0; j + 1 < R; ++j) {
+                ring[j][i] = ring[j + 1][i];
+                part[j][i] = part[j + 1][i];
+            }
+            ring[R - 1][i] = cen;
+            part[R - 1][i] = s;
+        }
+    }
+    __syncthreads();  // the next tile rewrites the LDS planes
+}
+
+__device__ __forceinline__ void bp_item(const BatchProblem *__restrict__ P, int np, int item, int sweep,
+                                        float *lds)
+{
+    int lo = 0, hi = np - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (P[mid].item0 <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchProblem &q = P[lo];
+    if (sweep >= q.iters) return;
+    const int local = item - q.item0;
+    if (q.g.r == 1) {
+        if (q.op == OP_THESIS) bp_tile<OP_THESIS, 1>(q, local, sweep, lds);
+        else bp_tile<OP_JACOBI, 1>(q, local, sweep, lds);
+    } else {
+        if (q.op == OP_THESIS) bp_tile<OP_THESIS, 2>(q, local, sweep, lds);
+        else bp_tile<OP_JACOBI, 2>(q, local, sweep, lds);
+    }
+}
+
+__global__ __launch_bounds__(kBpThreads) void k_bp_sweep(const BatchProblem *__restrict__ P, int np, int ntiles,
+                                                         int sweep)
+{
+    __shared__ float lds[2 * kBpLoads * kBpThreads];
+    const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
+    if (tile < ntiles) bp_item(P, np, tile, sweep, lds);
+}
+
+__global__ __launch_bounds__(kBpThreads) void k_bp_persistent(const BatchProblem *__restrict__ P, int np,
+                                                              int ntiles, int nsweeps)
+{
+    __shared__ float lds[2 * kBpLoads * kBpThreads];
+    const int b0 = (int)xcd_remap(blockIdx.x, gridDim.x);
+    for (int s = 0; s < nsweeps; ++s) {
+        for (int tile = b0; tile < ntiles; tile += (int)gridDim.x) bp_item(P, np, tile, s, lds);
+        if (s + 1 < nsweeps) cooperative_groups::this_grid().sync();
+    }
+}
+
 // ---------------------------------------------------------------- host side
 
 static const st_tile_rule *match_rule(const st_problem &p, const st_tile_rule *rules, int nrules)
@@ -115,8 +270,27 @@ static int valid_ty(int r, int ty)
     return ty >= 2 ? 2 : 1;
 }
 
+// The plane-chunk form takes a batch when every problem's frame of one row
+// plus 2R halo rows fits the LDS plane.
+static bool bp_fits(const st_problem &p)
+{
+    return p.radius >= 1 && p.radius <= 2 && p.nx <= kBpCells * kBpThreads &&
+           (1 + 2 * p.radius) * p.nx <= kBpLoads * kBpThreads;
+}
+
+// Frame rows: the rule's coarsen_y when given, else as many rows as keep the
+// frame within kBpCells cells per thread and the frame plus halo within the
+// LDS plane.
+static int bp_rows(const st_problem &p, int want)
+{
+    const int lim = std::min(kBpCells * kBpThreads / p.nx, kBpLoads * kBpThreads / p.nx - 2 * p.radius);
+    int f = want > 0 ? std::min(want, lim) : lim;
+    if (f > p.ny) f = p.ny;
+    return f < 1 ? 1 : f;
+}
+
 st_status build_batch(const st_problem *probs, int n, const st_tile_rule *rules, int nrules,
-                      std::vector<BatchProblem> &out, int &nitems, int &maxit)
+                      std::vector<BatchProblem> &out, int &nitems, int &maxit, bool plane_form)
 {
     out.clear();
     nitems = 0;
@@ -133,10 +307,15 @@ st_status build_batch(const st_problem *probs, int n, const st_tile_rule *rules,
         if (p.radius > 2) return fail(ST_EUNSUPPORTED, "batched sweeps are built for radius 1-2 (problem %d)", i);
         if (!p.u1 || !p.u2 || p.u1 == p.u2) return fail(ST_EINVAL, "problem %d needs two buffers", i);
         const st_tile_rule *rule = match_rule(p, rules, nrules);
-        int ty = rule ? rule->coarsen_y : 2;
-        int zc = rule ? rule->zchunk : 32;
-        ty = valid_ty(p.radius, ty);
-        if (zc <= 0) zc = p.nz;
+        int ty, zc;
+        if (plane_form) {
+            ty = bp_rows(p, rule ? rule->coarsen_y : 0);
+            zc = rule && rule->zchunk > 0 ? rule->zchunk : (p.nz <= 96 ? p.nz : 64);
+        } else {
+            ty = valid_ty(p.radius, rule ? rule->coarsen_y : 2);
+            zc = rule ? rule->zchunk : 32;
+            if (zc <= 0) zc = p.nz;
+        }
         BatchProblem b;
         b.g.nx = p.nx; b.g.ny = p.ny; b.g.nz = p.nz; b.g.r = p.radius;
         b.g.sy = p.nx; b.g.sz = (long long)p.nx * p.ny;
@@ -145,7 +324,7 @@ st_status build_batch(const st_problem *probs, int n, const st_tile_rule *rules,
         b.kc = p.op == ST_OP_THESIS ? (float)(6 * p.radius) : (float)w;
         b.iters = p.iterations;
         b.ty = ty;
-        b.tiles_x = (p.nx + 63) / 64;
+        b.tiles_x = plane_form ? 1 : (p.nx + 63) / 64;
         b.tiles_y = (p.ny + ty - 1) / ty;
         b.zchunk = zc;
         b.item0 = nitems;
@@ -190,9 +369,11 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
         }
         return ST_OK;
     }
+    bool plane_form = true;
+    for (int i = 0; i < n; ++i) plane_form = plane_form && bp_fits(probs[i]);
     std::vector<BatchProblem> table;
     int nitems = 0, maxit = 0;
-    st_status st = build_batch(probs, n, rules, nrules, table, nitems, maxit);
+    st_status st = build_batch(probs, n, rules, nrules, table, nitems, maxit, plane_form);
     if (st != ST_OK) return st;
     void *ws = nullptr;
     const size_t bytes = table.size() * sizeof(BatchProblem);
@@ -207,32 +388,36 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
         (void)hipEventCreate(&e0);
         (void)hipEventCreate(&e1);
     }
+    // work items: wave tiles (4 per workgroup) of the march form, workgroup
+    // tiles of the plane-chunk form
+    const int groups = plane_form ? nitems : (nitems + 3) / 4;
     if (mode == ST_BATCH_PERSISTENT) {
+        const void *fn = plane_form ? reinterpret_cast<const void *>(&k_bp_persistent)
+                                    : reinterpret_cast<const void *>(&k_batch_persistent);
         int per_cu = 0, dev = 0, cus = 0;
-        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void *>(&k_batch_persistent), 256, 0);
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0);
         if (e != hipSuccess) return hip_check(e, "occupancy");
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         int grid = per_cu * cus;
-        const int need = (nitems + 3) / 4;
-        if (grid > need) grid = need;
+        if (grid > groups) grid = groups;
         if (grid < 1) grid = 1;
         int np = n, ni = nitems, ns = maxit;
         void *args[] = {&P, &np, &ni, &ns};
         if (elapsed_ms) (void)hipEventRecord(e0, s);
-        e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_batch_persistent),
-                                       dim3(grid), dim3(256), args, 0, s);
+        e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(256), args, 0, s);
         if (e != hipSuccess) return hip_check(e, "batched cooperative launch");
     } else {
         // capture one launch per sweep, replay as one graph
-        const dim3 grid((nitems + 3) / 4), block(256);
+        const dim3 grid(groups), block(256);
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return hip_check(e, "begin capture");
-        for (int sw = 0; sw < maxit; ++sw)
-            hipLaunchKernelGGL(k_batch_sweep, grid, block, 0, s, P, n, nitems, sw);
+        for (int sw = 0; sw < maxit; ++sw) {
+            if (plane_form) hipLaunchKernelGGL(k_bp_sweep, grid, block, 0, s, P, n, nitems, sw);
+            else hipLaunchKernelGGL(k_batch_sweep, grid, block, 0, s, P, n, nitems, sw);
+        }
         e = hipStreamEndCapture(s, &graph);
         if (e != hipSuccess) return hip_check(e, "end capture");
         e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);

@@ -57,8 +57,9 @@ def tune(n=512, seed=0x5EED0004, iters=10, repeat=3, out=DEFAULT_OUT, verbose=Tr
             probs.append(dict(nx=nx, ny=ny, nz=nz, radius=rr, iterations=iters, op=st.OP_JACOBI,
                               u1=d1, u2=d2))
         stream.sync()
-        ty_vals = ["1", "2", "4"] if r == 1 else ["1", "2"]
-        values = {"TY": ty_vals, "ZCHUNK": ["8", "16", "32", "64", "0"]}
+        # plane-chunk form: TY = frame rows (0: as many as fit), ZCHUNK =
+        # planes per tile (0: the whole z extent)
+        values = {"TY": ["0", "4", "8", "16"], "ZCHUNK": ["16", "32", "64", "0"]}
 
         def score(v):
             rule = dict(radius=r, max_cells=bucket, coarsen_y=int(v["TY"]), zchunk=int(v["ZCHUNK"]))
