@@ -397,6 +397,8 @@ ROWS_CASES = [
     ((300, 19, 18), "f64", (1, 2, 3), {}),           # 2 waves per row at R=2,3
     ((1000, 13, 9), "f32", (1,), {}),                # 4 waves per row
     ((640, 17, 11), "f32", (2,), {}),                # 3 waves per row, a short last part
+    ((384, 21, 19), "f64", (4,), {}),                # frames shorter than 2R: two halo strips per wave
+    ((1024, 18, 12), "f32", (3, 4), {}),             # 4 waves per row, two halo strips per wave
 ]
 
 
@@ -428,8 +430,8 @@ def test_smem_register_rows_form(gpu, oracle, shape, dt, radii, knobs, op):
 
 def test_smem_register_rows_form_rejects_unsupported(gpu):
     st = gpu
-    for kw in (dict(nx=62, ny=40, nz=40), dict(nx=2052, ny=40, nz=40, radius=4), dict(dim=40, radius=5),
-               dict(nx=384, ny=20, nz=20, radius=4, dtype=st.F64), dict(nx=64, ny=40, nz=1, dimensions=2)):
+    for kw in (dict(nx=62, ny=40, nz=40), dict(nx=4100, ny=40, nz=40, radius=4), dict(dim=40, radius=5),
+               dict(nx=64, ny=40, nz=1, dimensions=2)):
         p = st.StencilParams.make(variant="smem_register", temporal_form=2, heuristic=1, op=1,
                                   iterations=2, **kw)
         with pytest.raises(st.StencilError):

@@ -155,6 +155,7 @@ static st_status resolve(const st_params *p, Cfg &c, bool mixed)
     c.form = p->temporal_form;
     c.kind = 0;
     c.wx = c.wy = 0;
+    c.hs = 1;
     c.zchunk = 0;
     c.lds = 0;
     c.gx = c.gy = c.gz = 1;

@@ -29,6 +29,7 @@ struct Cfg {
     int form;            // COOP form requested (0 auto, 1 LDS ring, 2 register queues)
     int kind;            // COOP form resolved
     int wx, wy;          // COOP register form: waves across x and y
+    int hs;              // SMEM_REGISTER row form: halo strips per wave
 };
 
 // Record a detail string for stencil_last_error() and return `s`.

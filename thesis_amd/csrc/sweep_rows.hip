@@ -81,23 +81,26 @@ __device__ __forceinline__ T dpp_next(T v)  // lane i <- lane i+1 (rotating)
 // leave 168, 512 leave 256.  Shapes above 250 are not built.
 enum : int { RM_SWEEP = 0, RM_CORR = 1, RM_DEFECT = 2 };
 
-template <int R, int CX, typename T, int MODE = RM_SWEEP>
+template <int R, int CX, typename T, int MODE = RM_SWEEP, int HS = 1>
 constexpr int rows_vgprs()
 {
-    return (9 * R + 3) * CX * (int)(sizeof(T) / 4) / 2 + 25 + (MODE == RM_CORR ? CX : 0);
+    return (9 * R + 3) * CX * (int)(sizeof(T) / 4) / 2 + 25 + (MODE == RM_CORR ? CX : 0) +
+           (HS - 1) * (CX * (int)(sizeof(T) / 4) + 8);
 }
 
-template <int R, int CX, typename T, int MODE = RM_SWEEP>
-constexpr bool rows_supported() { return rows_vgprs<R, CX, T, MODE>() <= 250; }
+template <int R, int CX, typename T, int MODE = RM_SWEEP, int HS = 1>
+constexpr bool rows_supported() { return rows_vgprs<R, CX, T, MODE, HS>() <= 250; }
 
-template <int R, int CX, typename T, int MODE = RM_SWEEP>
+template <int R, int CX, typename T, int MODE = RM_SWEEP, int HS = 1>
 constexpr int rows_max_threads()
 {
-    return rows_vgprs<R, CX, T, MODE>() <= 120 ? 1024 : rows_vgprs<R, CX, T, MODE>() <= 160 ? 768 : 512;
+    return rows_vgprs<R, CX, T, MODE, HS>() <= 120   ? 1024
+           : rows_vgprs<R, CX, T, MODE, HS>() <= 160 ? 768
+                                                     : 512;
 }
 
-template <int OP, int R, int CX, typename T, int MODE = RM_SWEEP, bool F16 = false>
-__global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE>())) void k_rows(
+template <int OP, int R, int CX, typename T, int MODE = RM_SWEEP, bool F16 = false, int HS = 1>
+__global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_rows(
     const void *__restrict__ in_, void *__restrict__ out_, const uint16_t *__restrict__ aux, RowsArgs a,
     T kc)
 {
@@ -126,13 +129,20 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE>())) void k_rows(
         xo[h] = act[h] ? x : a.nx - 4;  // quads past the row reload its last cells
         intq[h] = act[h] && x >= R && x + 4 <= a.nx - R;
     }
-    // halo strip of this wave (waves 0..2R*wx-1): row hr of the 2R halo rows,
-    // part hp of it
-    const bool has_halo = w < 2 * R * wx;
-    const int hr = w / wx, hp = w - hr * wx;
-    int hxo[H];
+    // halo strips of this wave: strip s = w + i*waves (i < HS) of the 2R*wx
+    // strips, row hr = s / wx of the 2R halo rows, part hp = s % wx of it
+    const int waves = blockDim.x >> 6;
+    bool has_halo[HS];
+    int hr[HS], hp[HS], hxo[HS][H];
 #pragma unroll
-    for (int h = 0; h < H; ++h) hxo[h] = min(256 * (hp * H + h) + 4 * lane, a.nx - 4);
+    for (int i = 0; i < HS; ++i) {
+        const int st = w + i * waves;
+        has_halo[i] = st < 2 * R * wx;
+        hr[i] = st / wx;
+        hp[i] = st - hr[i] * wx;
+#pragma unroll
+        for (int h = 0; h < H; ++h) hxo[i][h] = min(256 * (hp[i] * H + h) + 4 * lane, a.nx - 4);
+    }
     const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
 
     for (int tile = (int)bid; tile < a.ntiles; tile += (int)gridDim.x) {
@@ -143,9 +153,13 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE>())) void k_rows(
         const bool store_row = gy >= R && gy < a.ny - R;
         // halo rows: frame rows -R..-1 (LDS rows 0..R-1), wy..wy+R-1 (LDS
         // rows R+wy..)
-        const int hrow = hr < R ? hr : wy + hr;
-        const int hgy = hr < R ? fy0 - R + hr : fy0 + wy + (hr - R);
-        const int halo_off = min(max(hgy, 0), a.ny - 1) * a.nx;
+        int hrow[HS], halo_off[HS];
+#pragma unroll
+        for (int i = 0; i < HS; ++i) {
+            hrow[i] = hr[i] < R ? hr[i] : wy + hr[i];
+            const int hgy = hr[i] < R ? fy0 - R + hr[i] : fy0 + wy + (hr[i] - R);
+            halo_off[i] = min(max(hgy, 0), a.ny - 1) * a.nx;
+        }
         const int z0 = tzi * a.zchunk, z1 = min(z0 + a.zchunk, a.nz);
 
         auto load_row = [&](int k, int off, const int (&cols)[H], T (&d)[CX]) {
@@ -170,14 +184,16 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE>())) void k_rows(
         };
         const int k_out_lo = max(z0, R), k_out_hi = min(z1, a.nz - R);
 
-        T ring[R][CX], part[R][CX], pf[CX], ph[CX];
+        T ring[R][CX], part[R][CX], pf[CX], ph[HS][CX];
 #pragma unroll
         for (int i = 0; i < R; ++i)
 #pragma unroll
             for (int c = 0; c < CX; ++c) ring[i][c] = part[i][c] = (T)0;
         const int kin_lo = z0 - R, kin_hi = z1 + R;
         load_row(kin_lo, own_off, xo, pf);
-        if (has_halo) load_row(kin_lo, halo_off, hxo, ph);
+#pragma unroll
+        for (int i = 0; i < HS; ++i)
+            if (has_halo[i]) load_row(kin_lo, halo_off[i], hxo[i], ph[i]);
 
         for (int m = kin_lo; m < kin_hi; ++m) {
             T nw[CX];
@@ -191,18 +207,22 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE>())) void k_rows(
                 for (int j = 0; j < 4; ++j) q[j] = nw[4 * h + j];
                 buf[((R + wr) * QW + wxp * H + h) * 64 + lane] = q;
             }
-            if (has_halo) {
+#pragma unroll
+            for (int i = 0; i < HS; ++i) {
+                if (!has_halo[i]) continue;
 #pragma unroll
                 for (int h = 0; h < H; ++h) {
                     QV q;
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) q[j] = ph[4 * h + j];
-                    buf[(hrow * QW + hp * H + h) * 64 + lane] = q;
+                    for (int j = 0; j < 4; ++j) q[j] = ph[i][4 * h + j];
+                    buf[(hrow[i] * QW + hp[i] * H + h) * 64 + lane] = q;
                 }
             }
             if (m + 1 < kin_hi) {
                 load_row(m + 1, own_off, xo, pf);
-                if (has_halo) load_row(m + 1, halo_off, hxo, ph);
+#pragma unroll
+                for (int i = 0; i < HS; ++i)
+                    if (has_halo[i]) load_row(m + 1, halo_off[i], hxo[i], ph[i]);
             }
             // correction sweep: the defect at the cell this step completes
             // (plane m-R), loaded now and used after the barrier
@@ -376,55 +396,62 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE>())) void k_rows(
 
 // ---------------------------------------------------------------- host side
 
-template <int R, int CX, typename T, int MODE, bool F16>
+// Built instantiations: every (R, CX, T) of the sweep whose registers fit,
+// the 16-bit modes likewise, and two halo strips per wave (frames shorter
+// than 2R rows: rows spanning several waves) for the 4-cell sweeps.
+template <int R, int CX, typename T, int MODE, bool F16, int HS>
 static const void *rows_fn_t(int op)
 {
-    if constexpr (!rows_supported<R, CX, T, MODE>()) {
+    if constexpr (!rows_supported<R, CX, T, MODE, HS>() || (HS > 1 && (CX != 4 || MODE != RM_SWEEP))) {
         return nullptr;
     } else if constexpr (MODE != RM_SWEEP) {
-        return reinterpret_cast<const void *>(&k_rows<OP_JACOBI, R, CX, T, MODE, F16>);
+        return reinterpret_cast<const void *>(&k_rows<OP_JACOBI, R, CX, T, MODE, F16, HS>);
     } else {
-        return op == OP_THESIS ? reinterpret_cast<const void *>(&k_rows<OP_THESIS, R, CX, T>)
-                               : reinterpret_cast<const void *>(&k_rows<OP_JACOBI, R, CX, T>);
+        return op == OP_THESIS ? reinterpret_cast<const void *>(&k_rows<OP_THESIS, R, CX, T, MODE, F16, HS>)
+                               : reinterpret_cast<const void *>(&k_rows<OP_JACOBI, R, CX, T, MODE, F16, HS>);
     }
 }
 
 template <int R, typename T, int MODE, bool F16>
-static const void *rows_fn_r(int op, int cx, int *threads)
+static const void *rows_fn_r(int op, int cx, int hs, int *threads)
 {
     if (cx == 8) {
         *threads = rows_max_threads<R, 8, T, MODE>();
-        return rows_fn_t<R, 8, T, MODE, F16>(op);
+        return rows_fn_t<R, 8, T, MODE, F16, 1>(op);
+    }
+    if (hs == 2) {
+        *threads = rows_max_threads<R, 4, T, MODE, 2>();
+        return rows_fn_t<R, 4, T, MODE, F16, 2>(op);
     }
     *threads = rows_max_threads<R, 4, T, MODE>();
-    return rows_fn_t<R, 4, T, MODE, F16>(op);
+    return rows_fn_t<R, 4, T, MODE, F16, 1>(op);
 }
 
 template <typename T, int MODE, bool F16>
-static const void *rows_fn_d(int op, int r, int cx, int *threads)
+static const void *rows_fn_d(int op, int r, int cx, int hs, int *threads)
 {
     switch (r) {
-    case 1: return rows_fn_r<1, T, MODE, F16>(op, cx, threads);
-    case 2: return rows_fn_r<2, T, MODE, F16>(op, cx, threads);
-    case 3: return rows_fn_r<3, T, MODE, F16>(op, cx, threads);
-    default: return rows_fn_r<4, T, MODE, F16>(op, cx, threads);
+    case 1: return rows_fn_r<1, T, MODE, F16>(op, cx, hs, threads);
+    case 2: return rows_fn_r<2, T, MODE, F16>(op, cx, hs, threads);
+    case 3: return rows_fn_r<3, T, MODE, F16>(op, cx, hs, threads);
+    default: return rows_fn_r<4, T, MODE, F16>(op, cx, hs, threads);
     }
 }
 
 // mode RM_SWEEP: c.dtype F32/F64; RM_CORR / RM_DEFECT: FP32 arithmetic on
-// 16-bit storage (f16 selects FP16, else BF16)
-static const void *rows_fn_mode(const Cfg &c, int cx, int mode, bool f16, int *threads)
+// 16-bit storage (f16 selects FP16, else BF16); hs halo strips per wave
+static const void *rows_fn_mode(const Cfg &c, int cx, int mode, bool f16, int *threads, int hs = 1)
 {
     *threads = 0;
     if (c.r < 1 || c.r > 4) return nullptr;
     if (mode == RM_CORR)
-        return f16 ? rows_fn_d<float, RM_CORR, true>(c.op, c.r, cx, threads)
-                   : rows_fn_d<float, RM_CORR, false>(c.op, c.r, cx, threads);
+        return f16 ? rows_fn_d<float, RM_CORR, true>(c.op, c.r, cx, hs, threads)
+                   : rows_fn_d<float, RM_CORR, false>(c.op, c.r, cx, hs, threads);
     if (mode == RM_DEFECT)
-        return f16 ? rows_fn_d<float, RM_DEFECT, true>(c.op, c.r, cx, threads)
-                   : rows_fn_d<float, RM_DEFECT, false>(c.op, c.r, cx, threads);
-    if (c.dtype == ST_F64) return rows_fn_d<double, RM_SWEEP, false>(c.op, c.r, cx, threads);
-    return rows_fn_d<float, RM_SWEEP, false>(c.op, c.r, cx, threads);
+        return f16 ? rows_fn_d<float, RM_DEFECT, true>(c.op, c.r, cx, hs, threads)
+                   : rows_fn_d<float, RM_DEFECT, false>(c.op, c.r, cx, hs, threads);
+    if (c.dtype == ST_F64) return rows_fn_d<double, RM_SWEEP, false>(c.op, c.r, cx, hs, threads);
+    return rows_fn_d<float, RM_SWEEP, false>(c.op, c.r, cx, hs, threads);
 }
 
 // Lane layout: 4 cells per lane and one wave per row up to 256 columns; up
@@ -435,8 +462,9 @@ static const void *rows_fn_mode(const Cfg &c, int cx, int mode, bool f16, int *t
 // Returns false when no built kernel fits (a frame must hold the 2R halo
 // strips: rows >= 2R).
 static bool rows_layout(const Cfg &c, int mode, bool f16, int *cx, int *wx, int *threads,
-                        const void **fn)
+                        const void **fn, int *hs)
 {
+    *hs = 1;
     if (c.nx % 4 || c.nx < 8 || c.r < 1 || c.r > 4) return false;
     int t8 = 0;
     const void *f8 = c.nx <= 512 ? rows_fn_mode(c, 8, mode, f16, &t8) : nullptr;
@@ -456,33 +484,40 @@ static bool rows_layout(const Cfg &c, int mode, bool f16, int *cx, int *wx, int 
     *fn = *cx == 8 ? f8 : rows_fn_mode(c, 4, mode, f16, threads);
     if (*cx == 8) *threads = t8;
     if (!*fn) return false;
-    return (*threads / 64) / *wx >= 2 * c.r;
+    if ((*threads / 64) / *wx >= 2 * c.r) return true;
+    if (*cx != 4) return false;
+    // frames shorter than 2R rows: each wave loads two halo strips
+    *fn = rows_fn_mode(c, 4, mode, f16, threads, 2);
+    *hs = 2;
+    return *fn && (*threads / 64) / *wx >= 1 && 2 * (*threads / 64) >= 2 * c.r * *wx;
 }
 
 bool rows_eligible(const Cfg &c)
 {
     if (!((c.dtype == ST_F32 || c.dtype == ST_F64) && c.dims == 3)) return false;
-    int cx, wx, threads;
+    int cx, wx, threads, hs;
     const void *fn;
-    return rows_layout(c, RM_SWEEP, false, &cx, &wx, &threads, &fn);
+    return rows_layout(c, RM_SWEEP, false, &cx, &wx, &threads, &fn, &hs);
 }
 
 static st_status rows_resolve_mode(Cfg &c, int mode, bool f16)
 {
-    int cx = 0, wx = 1, threads = 0;
+    int cx = 0, wx = 1, threads = 0, hs = 1;
     const void *fn = nullptr;
-    if (c.dims != 3 || !rows_layout(c, mode, f16, &cx, &wx, &threads, &fn))
+    if (c.dims != 3 || !rows_layout(c, mode, f16, &cx, &wx, &threads, &fn, &hs))
         return fail(ST_EUNSUPPORTED, "row-streaming form needs a 3D F32/F64 grid, radius 1..4 and nx a "
-                                     "multiple of 4 whose rows fit a workgroup (at least 2R rows per frame)");
+                                     "multiple of 4 whose rows fit a workgroup with its halo strips");
     const int rows_max = threads / 64 / wx;
+    // every one of the 2R*wx halo strips needs a wave (hs strips per wave)
+    const int rows_min = (2 * c.r * wx + hs * wx - 1) / (hs * wx);
     int wy = rows_max;
     if (!c.heuristic && c.by > 0) {
         // block_y = rows per frame
-        if (c.by < 2 * c.r || c.by > rows_max)
-            return fail(ST_EINVAL, "row-streaming form: block_y (rows per frame) must be %d..%d", 2 * c.r, rows_max);
+        if (c.by < rows_min || c.by > rows_max)
+            return fail(ST_EINVAL, "row-streaming form: block_y (rows per frame) must be %d..%d", rows_min, rows_max);
         wy = c.by;
     } else if (wy > c.ny) {
-        wy = c.ny < 2 * c.r ? 2 * c.r : c.ny;
+        wy = c.ny < rows_min ? rows_min : c.ny;
     }
     const int es = (c.dtype == ST_F64 && mode == RM_SWEEP) ? 8 : 4;
     c.lds = 2 * (wy + 2 * c.r) * wx * (cx / 4) * 64 * 4 * es;
@@ -504,6 +539,7 @@ static st_status rows_resolve_mode(Cfg &c, int mode, bool f16)
     c.kind = 4;
     c.cx = cx;
     c.cy = 1;
+    c.hs = hs;
     c.wx = wx;
     c.wy = wy;
     c.bx = c.nx;
@@ -548,7 +584,7 @@ static st_status rows_launch(const Cfg &c, const void *fn, const void *in, void 
 st_status launch_rows(const Cfg &c, const void *in, void *out, hipStream_t s)
 {
     int threads = 0;
-    const void *fn = rows_fn_mode(c, c.cx, RM_SWEEP, false, &threads);
+    const void *fn = rows_fn_mode(c, c.cx, RM_SWEEP, false, &threads, c.hs);
     if (!fn || 64 * c.wx * c.wy > threads) return fail(ST_EUNSUPPORTED, "row-streaming kernel missing");
     return rows_launch(c, fn, in, out, nullptr, s);
 }
@@ -577,7 +613,7 @@ st_status launch_rows_mixed(const Cfg &c, int defect, bool f16, const void *in, 
     st_status st = rows_resolve_mode(rc, mode, f16);
     if (st != ST_OK) return st;
     int threads = 0;
-    const void *fn = rows_fn_mode(rc, rc.cx, mode, f16, &threads);
+    const void *fn = rows_fn_mode(rc, rc.cx, mode, f16, &threads, rc.hs);
     if (!fn || 64 * rc.wx * rc.wy > threads) return fail(ST_EUNSUPPORTED, "row-streaming mixed kernel missing");
     return rows_launch(rc, fn, in, out, d, s);
 }
