@@ -1,5 +1,6 @@
 """The result databases filled on the GPU through the C ABI."""
 import json
+import os
 
 import pytest
 
@@ -27,3 +28,17 @@ def test_evaluate_and_autotune_on_device(gpu, tmp_path):
     R.evaluate_stencils(out, [3], {3: [64]}, ["smem"], [1], [1], [8], host="box", repeat=2, tune_db=tune)
     db = json.load(open(out))
     assert len(db["3"]["64"]["1_gpus_smem"]["1"]["8"]["box"]["autotune"]["time"]) == 2
+
+
+@pytest.mark.skipif(not os.environ.get("THESIS_PROFILE_TESTS"),
+                    reason="runs rocprofv3 counter passes; opt in with THESIS_PROFILE_TESTS=1")
+def test_rocprof_metrics_on_device(gpu, tmp_path):
+    from thesis_amd import results_db as R
+    print("[case] rocprof metrics smem_register 128^3", flush=True)
+    cmd = R.cli_command(3, 128, "smem_register", 1, 8)
+    m = R.rocprof_metrics(cmd, str(tmp_path), timeout_s=90)
+    for k in ("dram_read_throughput", "dram_write_throughput", "achieved_occupancy", "shared_ld_bank_conflict",
+              "l2_tex_hit_rate"):
+        assert k in m, (k, m)
+    assert float(m["dram_read_throughput"].rstrip("GB/s")) > 0
+    assert 0 < float(m["achieved_occupancy"]) <= 1

@@ -93,3 +93,43 @@ def test_params_follow_the_reference_knobs():
     assert (p.heuristic, p.block_x, p.block_y, p.block_z) == (0, 32, 4, 2)
     q = R.make_params(2, 256, "base", 1, 64)
     assert (q.nx, q.ny, q.nz, q.dimensions, q.heuristic) == (256, 256, 1, 2, 1)
+
+
+def test_profile_layout_and_skip(tmp_path):
+    out = str(tmp_path / "results.json")
+    seen = []
+
+    def metrics(cmd):
+        seen.append(cmd)
+        return {"dram_read_throughput": "123.4000GB/s", "shared_ld_bank_conflict": "0.5000%",
+                "achieved_occupancy": "0.812500"}
+
+    R.profile_stencils(out, [3], {3: [64]}, ["base", "smem"], [1], [1, 2], [8], host="box", metrics=metrics)
+    db = json.load(open(out))
+    e = dotted_get(db, "3.64.1_gpus_smem.2.8.box.heuristic")
+    assert e["dram_read_throughput"] == "123.4000GB/s" and e["achieved_occupancy"] == "0.812500"
+    assert len(seen) == 4
+    cmd = seen[0]
+    assert cmd[0].endswith("stencil") and cmd[cmd.index("--variant") + 1] == "base"
+    assert cmd[cmd.index("--radius") + 1] == "1" and cmd[cmd.index("--heuristic") + 1] == "1"
+    # entries with metrics are not profiled again; "time" beside them survives
+    R.evaluate_stencils(out, [3], {3: [64]}, ["base"], [1], [1], [8], host="box", repeat=2,
+                        timer=lambda p, rep: [1.0] * rep)
+    R.profile_stencils(out, [3], {3: [64]}, ["base", "smem"], [1], [1, 2], [8], host="box",
+                       metrics=lambda cmd: (_ for _ in ()).throw(AssertionError("profiled twice")))
+    db = json.load(open(out))
+    e = dotted_get(db, "3.64.1_gpus_base.1.8.box.heuristic")
+    assert e["time"] == [1.0, 1.0] and e["shared_ld_bank_conflict"] == "0.5000%"
+
+
+def test_read_counters_skips_helper_kernels(tmp_path):
+    p = tmp_path / "p_counter_collection.csv"
+    p.write_text(
+        '"Correlation_Id","Dispatch_Id","Kernel_Name","Scratch_Size","Counter_Name","Counter_Value",'
+        '"Start_Timestamp","End_Timestamp"\n'
+        '1,1,"void st::k_init<float>(float*)",0,"FETCH_SIZE",10.0,1000,2000\n'
+        '2,2,"void st::k_rows<1, 1, 4, float, 0, false>(void const*)",0,"FETCH_SIZE",1024.0,0,1000000\n'
+        '3,3,"void st::k_rows<1, 1, 4, float, 0, false>(void const*)",0,"FETCH_SIZE",2048.0,0,2000000\n')
+    got = R._read_counters(str(p))
+    assert [v for v, _, _ in got["FETCH_SIZE"]] == [1024.0, 2048.0]
+    assert got["FETCH_SIZE"][1][1] == pytest.approx(2e-3)

@@ -38,7 +38,8 @@ static void usage()
                  "               [--coarsen-x C] [--coarsen-y C] [--smem] [--padded] [--register] [--coop]\n"
                  "               [--variant base|smem|smem_padded|smem_register|coop|auto]\n"
                  "               [--op thesis|jacobi] [--dtype f32|f64] [--init reference|seeded]\n"
-                 "               [--seed S] [--temporal-steps T] [--smem-pad P] [--debug] [--save]\n");
+                 "               [--seed S] [--temporal-steps T] [--form F] [--launch-per-pass]\n"
+                 "               [--smem-pad P] [--debug] [--save]\n");
 }
 
 static bool file_exists(const std::string &f)
@@ -83,6 +84,7 @@ int main(int argc, char **argv)
         else if (a == "--heuristic") p.heuristic = std::atoi(next());
         else if (a == "--temporal-steps") p.temporal_steps = std::atoi(next());
         else if (a == "--launch-per-pass") p.launch_per_pass = 1;
+        else if (a == "--form") p.temporal_form = std::atoi(next());
         else if (a == "--smem") smem = true;
         else if (a == "--padded") padded = true;
         else if (a == "--register") reg = true;

@@ -34,6 +34,11 @@ Command line::
         --dimensions 3 --dims 256 --versions base smem smem_padded --depths 1 2 4 \\
         --unrolls 1 2 --iterations 8 --repeat 30 [--autotune results/results_autotune.json]
     python -m thesis_amd.results_db autotune --out results/results_autotune.json ...
+    python -m thesis_amd.results_db profile --out results/results_stencils.json ...
+
+The profile mode adds, beside "time", the hardware metrics the reference
+collected with nvprof (scripts/profile_configuration.sh), under the same
+names, from rocprofv3 counters (see rocprof_metrics for the mapping).
 """
 from __future__ import annotations
 
@@ -287,6 +292,168 @@ def autotune_stencils(out: str, dimensions: Iterable[int] = (3,), dims: Optional
     return db
 
 
+# ---------------------------------------------------------------- profiling
+
+# rocprofv3 counter passes (one run each: rocprofv3 does not split a request
+# over passes) and the calibration of the memory counters against the
+# library's streaming copy kernel (profiles/traffic_512cubed_r1.md: FETCH_SIZE
+# counts half of the bytes read on this GPU, WRITE_SIZE all of them).
+PROFILE_PASSES = (("FETCH_SIZE",), ("WRITE_SIZE",),
+                  ("OccupancyPercent", "LDSBankConflict", "VALUBusy", "L2CacheHit"))
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+_HELPER_KERNELS = ("k_init", "k_shell_eq", "k_copy_f32", "k_fill16")
+
+
+def cli_command(dimension: int, dim: int, version: str, radius: int, iterations: int, unroll: int = 1,
+                block=None, form: int = 0) -> List[str]:
+    """The `stencil` CLI line of one reference configuration (the knobs of
+    scripts/set_run_configuration.sh and profile_configuration.sh)."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    cmd = [os.path.join(here, "bin", "stencil"), "--dim", str(dim), "--dimensions", str(dimension),
+           "--radius", str(radius), "--iterations", str(iterations), "--variant", version,
+           "--coarsen-x", str(unroll)]
+    if form:
+        cmd += ["--form", str(form)]
+    if block is None:
+        cmd += ["--heuristic", "1"]
+    else:
+        cmd += ["--heuristic", "0", "--block-x", str(block[0]), "--block-y", str(block[1]),
+                "--block-z", str(block[2] if dimension == 3 else 1)]
+    return cmd
+
+
+def _read_counters(csv_path: str):
+    """{counter: [(value, duration_s, scratch_bytes)] over the sweep kernels'
+    dispatches} from a rocprofv3 counter_collection.csv."""
+    import csv
+
+    out: Dict[str, list] = {}
+    with open(csv_path) as f:
+        for r in csv.DictReader(f):
+            name = r.get("Kernel_Name", "")
+            if any(h in name for h in _HELPER_KERNELS):
+                continue
+            dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9
+            out.setdefault(r["Counter_Name"], []).append(
+                (float(r["Counter_Value"]), dur, int(float(r.get("Scratch_Size", 0) or 0))))
+    return out
+
+
+def rocprof_metrics(cmd: List[str], workdir: str, timeout_s: int = 120) -> Dict[str, str]:
+    """Run `cmd` under rocprofv3 once per counter pass and map the counters
+    to the metric names of the reference's nvprof database entries
+    (scripts/profile_configuration.sh, read by scripts/plot_measurements.py,
+    which keeps only the number of each string):
+
+      dram_read_throughput / dram_write_throughput   calibrated FETCH_SIZE /
+                                                     WRITE_SIZE per second
+      achieved_occupancy      OccupancyPercent / 100 (fraction of 32 waves/CU)
+      shared_ld_bank_conflict LDSBankConflict: % of GPU time LDS stalled on
+                              bank conflicts (nvprof counted conflicts)
+      l2_tex_hit_rate         L2CacheHit (%)
+      local_load_throughput, local_store_throughput
+                              0 when the kernels use no scratch (spill) memory
+    plus AMD-named values (valu_busy_pct, kernel_ms).  Counter values are
+    averaged over the sweep kernels' dispatches."""
+    import glob
+    import shutil
+    import subprocess
+
+    env = dict(os.environ, TMPDIR="/tmp")
+    got: Dict[str, list] = {}
+    for i, counters in enumerate(PROFILE_PASSES):
+        d = os.path.join(workdir, f"pass{i}")
+        shutil.rmtree(d, ignore_errors=True)
+        prof = ["rocprofv3", "--pmc", *counters, "--output-format", "csv", "-d", d, "-o", "p", "--", *cmd]
+        r = subprocess.run(prof, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           timeout=timeout_s)
+        files = glob.glob(os.path.join(d, "**", "p_counter_collection.csv"), recursive=True)
+        if not files:
+            raise RuntimeError(f"rocprofv3 pass {counters} wrote no counters (exit {r.returncode}): "
+                               f"{r.stdout.decode(errors='replace')[-400:]}")
+        got.update(_read_counters(files[0]))
+
+    def mean(name, idx=0):
+        v = got.get(name)
+        if not v:
+            return None
+        return sum(x[idx] for x in v) / len(v)
+
+    out: Dict[str, str] = {}
+    fetch, write = got.get("FETCH_SIZE"), got.get("WRITE_SIZE")
+    if fetch:
+        gbs = sum(v * 1024.0 * FETCH_SCALE / t for v, t, _ in fetch) / len(fetch) / 1e9
+        out["dram_read_throughput"] = f"{gbs:.4f}GB/s"
+        out["kernel_ms"] = f"{1e3 * sum(t for _, t, _ in fetch) / len(fetch):.6f}"
+    if write:
+        gbs = sum(v * 1024.0 * WRITE_SCALE / t for v, t, _ in write) / len(write) / 1e9
+        out["dram_write_throughput"] = f"{gbs:.4f}GB/s"
+    occ = mean("OccupancyPercent")
+    if occ is not None:
+        out["achieved_occupancy"] = f"{occ / 100.0:.6f}"
+    lds = mean("LDSBankConflict")
+    if lds is not None:
+        out["shared_ld_bank_conflict"] = f"{lds:.4f}%"
+    l2 = mean("L2CacheHit")
+    if l2 is not None:
+        out["l2_tex_hit_rate"] = f"{l2:.2f}%"
+    valu = mean("VALUBusy")
+    if valu is not None:
+        out["valu_busy_pct"] = f"{valu:.2f}%"
+    scratch = [x[2] for v in got.values() for x in v]
+    if scratch and max(scratch) == 0:
+        out["local_load_throughput"] = "0.000000B/s"
+        out["local_store_throughput"] = "0.000000B/s"
+    return out
+
+
+def profile_stencils(out: str, dimensions: Iterable[int] = (3,), dims: Optional[Dict[int, List[int]]] = None,
+                     versions: Iterable[str] = ("base", "smem", "smem_padded"),
+                     unrolls: Iterable[int] = (1,), depths: Iterable[int] = (1, 2, 4, 8, 16),
+                     iterations: Iterable[int] = (8,), host: Optional[str] = None, gpus: int = 1,
+                     tune_db: Optional[str] = None, workdir: str = "/tmp/thesis_amd_profile",
+                     metrics: Optional[Callable] = None, verbose: bool = False) -> dict:
+    """Fill the metric entries of the measurement database `out` beside
+    "time" (scripts/evaluate_stencils.py:112-173): an entry that already has
+    "shared_ld_bank_conflict" is skipped, as the reference checks.
+    `metrics(cmd) -> {name: str}` defaults to rocprof_metrics."""
+    db = _load(out)
+    tdb = _load(tune_db) if tune_db else {}
+    config = "autotune" if tune_db else "heuristic"
+    host = host or socket.gethostname().split(".")[0]
+    dims = dims or {3: [256, 1024], 2: [4096, 32768]}
+    metrics = metrics or (lambda cmd: rocprof_metrics(cmd, workdir))
+    for dimension in dimensions:
+        for dim in dims[dimension]:
+            for version in versions:
+                for unroll in unrolls:
+                    v = version_key(version, unroll, gpus)
+                    for depth in depths:
+                        for it in iterations:
+                            keys = [dimension, dim, v, depth, it, host, config]
+                            if deep_get(db, keys + ["shared_ld_bank_conflict"]) is not None:
+                                continue
+                            block = None
+                            if tune_db:
+                                bd = deep_get(tdb, [dimension, dim, tune_key(version, unroll), depth])
+                                if not bd:
+                                    continue
+                                block = (int(bd["BLOCK_X"]), int(bd["BLOCK_Y"]), int(bd.get("BLOCK_Z", 1)))
+                            cmd = cli_command(dimension, dim, version, depth, it, unroll, block)
+                            try:
+                                vals = metrics(cmd)
+                            except (RuntimeError, OSError, ValueError) as e:
+                                if verbose:
+                                    print(f"[results_db] {keys}: profile failed ({e})", flush=True)
+                                continue
+                            _node(db, keys).update(vals)
+                            _save(db, out)
+                            if verbose:
+                                print(f"[results_db] {keys}: {vals}", flush=True)
+    _save(db, out)
+    return db
+
+
 def _wave64_2d(valuation: Dict[str, str]) -> bool:
     bx, by = int(valuation["BLOCK_X"]), int(valuation["BLOCK_Y"])
     t = bx * by
@@ -299,7 +466,7 @@ def main(argv=None) -> int:
     import argparse
 
     ap = argparse.ArgumentParser(prog="python -m thesis_amd.results_db")
-    ap.add_argument("mode", choices=["evaluate", "autotune"])
+    ap.add_argument("mode", choices=["evaluate", "autotune", "profile"])
     ap.add_argument("--out", required=True)
     ap.add_argument("--dimensions", type=int, nargs="+", default=[3])
     ap.add_argument("--dims", type=int, nargs="+", default=None,
@@ -317,6 +484,9 @@ def main(argv=None) -> int:
     if a.mode == "evaluate":
         evaluate_stencils(a.out, a.dimensions, dims, a.versions, a.unrolls, a.depths, a.iterations,
                           a.host, a.repeat, tune_db=a.autotune, verbose=True)
+    elif a.mode == "profile":
+        profile_stencils(a.out, a.dimensions, dims, a.versions, a.unrolls, a.depths, a.iterations,
+                         a.host, tune_db=a.autotune, verbose=True)
     else:
         autotune_stencils(a.out, a.dimensions, dims, a.versions, a.unrolls, a.depths, a.iterations[0],
                           a.repeat, csv_dir=a.csv_dir, verbose=True)
