@@ -21,6 +21,7 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "st_device.hpp"
@@ -41,6 +42,20 @@ struct BatchProblem {
     float *u1, *u2;
 };
 
+// Index of the problem that owns a flat-list item: binary search over the
+// problems' first items, a compact int array (a few KB for hundreds of
+// problems, so the searches hit the scalar cache).
+__device__ __forceinline__ int find_problem(const int *__restrict__ starts, int np, int item)
+{
+    int lo = 0, hi = np - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 template <int R, int TY>
 __device__ __forceinline__ void batch_tile(const BatchProblem &q, int local, int sweep)
 {
@@ -58,16 +73,10 @@ __device__ __forceinline__ void batch_tile(const BatchProblem &q, int local, int
 
 // Advance the wave's item by one sweep.  Items past a problem's iteration
 // count do nothing.
-__device__ __forceinline__ void batch_item(const BatchProblem *__restrict__ P, int np, int item,
-                                           int sweep)
+__device__ __forceinline__ void batch_item(const BatchProblem *__restrict__ P, const int *__restrict__ starts,
+                                           int np, int item, int sweep)
 {
-    int lo = 0, hi = np - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (P[mid].item0 <= item) lo = mid;
-        else hi = mid - 1;
-    }
-    const BatchProblem &q = P[lo];
+    const BatchProblem &q = P[find_problem(starts, np, item)];
     if (sweep >= q.iters) return;
     const int local = item - q.item0;
     const int r = q.g.r, ty = q.ty;
@@ -81,44 +90,57 @@ __device__ __forceinline__ void batch_item(const BatchProblem *__restrict__ P, i
     }
 }
 
-__global__ __launch_bounds__(256) void k_batch_sweep(const BatchProblem *__restrict__ P, int np,
+__global__ __launch_bounds__(256) void k_batch_sweep(const BatchProblem *__restrict__ P,
+                                                     const int *__restrict__ starts, int np,
                                                      int nitems, int sweep)
 {
     const int item = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     if (item >= nitems) return;
-    batch_item(P, np, item, sweep);
+    batch_item(P, starts, np, item, sweep);
 }
 
 __global__ __launch_bounds__(256) void k_batch_persistent(const BatchProblem *__restrict__ P,
+                                                          const int *__restrict__ starts,
                                                           int np, int nitems, int nsweeps)
 {
     const int wpb = blockDim.x >> 6;
     const int nw = gridDim.x * wpb;
     const int w0 = blockIdx.x * wpb + (threadIdx.x >> 6);
     for (int s = 0; s < nsweeps; ++s) {
-        for (int item = w0; item < nitems; item += nw) batch_item(P, np, item, s);
+        for (int item = w0; item < nitems; item += nw) batch_item(P, starts, np, item, s);
         if (s + 1 < nsweeps) cooperative_groups::this_grid().sync();
     }
 }
 
 // ---------------------------------------------------------------- plane-chunk form
 //
-// A workgroup of 256 threads owns a tile: frame rows [y0, y0+F) of one
+// A workgroup of 512 threads owns a tile: frame rows [y0, y0+F) of one
 // problem over a z chunk.  A frame's rows of one plane, with R halo rows on
 // each side, are one contiguous run of memory, so every plane is fetched
 // with fully coalesced loads of consecutive floats (any nx), prefetched
-// into registers one plane ahead and written to a double-buffered LDS
-// plane (two planes in flight).  Each thread owns up to kBpCells cells of the frame (t, t+256, ...,
+// into registers two planes ahead and written to a double-buffered LDS
+// plane.  Each thread owns up to kBpCells cells of the frame (t, t+512, ...,
 // contiguous in memory as well); their x and y neighbours are read from the
 // LDS plane and z runs through a ring of R planes and R partial sums, in the
 // reference's order, as in sweep_rows.hip.
-constexpr int kBpThreads = 256;
-constexpr int kBpCells = 8;    // owned cells per thread: frame core <= 2048 cells
-constexpr int kBpLoads = 16;   // prefetched floats per thread: frame + halo <= 4096
+constexpr int kBpThreads = 512;
+constexpr int kBpCells = 4;    // owned cells per thread at R = 1: frame core <= 2048 cells
+// owned cells per thread by radius (R = 2 keeps half, for registers)
+__host__ __device__ constexpr int bp_cells(int r) { return r == 1 ? kBpCells : kBpCells / 2; }
+constexpr int kBpLoads = 8;    // prefetched floats per thread: frame + halo <= 4096
+// LDS floats per plane buffer: the last load slot's lanes past the window
+// write into the padding
+constexpr int kBpStride = kBpLoads * kBpThreads + kBpThreads;
+// LDS array: cells past the frame (up to bp_cells(R) * kBpThreads) still
+// read their y neighbours, at most 2R * nx + bp_cells(R) * kBpThreads - 1 <
+// 4800 floats into a buffer since (1 + 2R) * nx <= 4096 (R = 1: 2730 + 2047,
+// R = 2: 3276 + 1023)
+constexpr int kBpLds = kBpStride + 4800;
 
 template <int OP, int R>
 __device__ __forceinline__ void bp_tile(const BatchProblem &q, int local, int sweep, float *lds)
 {
+    constexpr int NC = bp_cells(R);
     const int t = threadIdx.x;
     const int nx = q.g.nx, ny = q.g.ny, nz = q.g.nz;
     const long long sz = q.g.sz;
@@ -132,60 +154,81 @@ __device__ __forceinline__ void bp_tile(const BatchProblem &q, int local, int sw
     // contiguous load window of a plane: rows [rlo, rhi) of the domain
     const int rlo = max(y0 - R, 0), rhi = min(y0 + fr + R, ny);
     const int nload = (rhi - rlo) * nx;
+    const int nslot = (nload + kBpThreads - 1) / kBpThreads;  // load slots in use
     const int ldoff = (rlo - (y0 - R)) * nx;  // LDS offset of row rlo
-    const int plane = (fr + 2 * R) * nx;      // LDS floats per buffer
 
     // owned cells: interior mask (x, y, plus the z test per plane)
-    unsigned own = 0, intr = 0;
+    unsigned intr = 0;
 #pragma unroll
-    for (int i = 0; i < kBpCells; ++i) {
+    for (int i = 0; i < NC; ++i) {
         const int c = t + i * kBpThreads;
         if (c < ncore) {
-            own |= 1u << i;
             const int r = c / nx, x = c - r * nx;
             const int y = y0 + r;
             if (x >= R && x < nx - R && y >= R && y < ny - R) intr |= 1u << i;
         }
     }
 
-    float ring[R][kBpCells], part[R][kBpCells], pf[kBpLoads], pf2[kBpLoads];
+    float ring[R][NC], part[R][NC], pf[kBpLoads], pf2[kBpLoads];
 #pragma unroll
     for (int j = 0; j < R; ++j)
 #pragma unroll
-        for (int i = 0; i < kBpCells; ++i) ring[j][i] = part[j][i] = 0.0f;
+        for (int i = 0; i < NC; ++i) ring[j][i] = part[j][i] = 0.0f;
 
-    auto load_plane = [&](int k, float (&d)[kBpLoads]) {
+    // Plane loads are raw buffer loads over the input buffer: slots past the
+    // window get an offset past the buffer and return 0 without touching
+    // memory, lanes of the last slot past the window read the following
+    // floats (or 0 past the buffer's end) into the LDS padding.  Every slot
+    // is issued and written to LDS, so the compiler's wait counts stay exact.
+    const __amdgpu_buffer_rsrc_t rin =
+        __builtin_amdgcn_make_buffer_rsrc((void *)in, (short)0, (int)(sz * nz * 4), 0x00020000);
+    // Results go out through raw buffer stores as well: a cell that is not
+    // written gets an offset past the buffer and the store is dropped, so
+    // every step issues the same number of stores.
+    const __amdgpu_buffer_rsrc_t rout =
+        __builtin_amdgcn_make_buffer_rsrc((void *)out, (short)0, (int)(sz * nz * 4), 0x00020000);
+    // A plane past the chunk (valid false) is issued all past the buffer.
+    // Bit 31 of an offset (past any buffer this form takes) marks a load
+    // that is off; bit arithmetic rather than selects keeps the loads free
+    // of branches.
+    unsigned slot_off[kBpLoads];
+#pragma unroll
+    for (int i = 0; i < kBpLoads; ++i) slot_off[i] = (unsigned)(i >= nslot) << 31;
+    auto load_plane = [&](int k, float (&d)[kBpLoads], bool valid) {
         k = min(max(k, 0), nz - 1);
-        const float *p = in + (long long)k * sz + (long long)rlo * nx;
+        const unsigned base = ((unsigned)(k * sz + (long long)rlo * nx) * 4u) | ((unsigned)!valid << 31);
 #pragma unroll
         for (int i = 0; i < kBpLoads; ++i) {
-            const int j = t + i * kBpThreads;
-            if (j < nload) d[i] = p[j];
+            const unsigned off = (base | slot_off[i]) + (unsigned)(t + i * kBpThreads) * 4u;
+            d[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rin, (int)off, 0, 0));
         }
     };
-    // two planes in flight: pf holds plane m when step m starts, pf2 plane
-    // m+1
+    // One step consumes the plane in `pa` (LDS buffer m & 1) and refills
+    // `pa` with plane m + 2; the loop alternates pf and pf2, so every load
+    // has two steps to land and no register copies force an early wait.
+    // Every step issues the same memory instructions (16 loads, NC stores,
+    // masked by offset), so the compiler waits for exactly the plane a step
+    // consumes.
     const int kin_lo = z0 - R, kin_hi = z1 + R;
-    load_plane(kin_lo, pf);
-    if (kin_lo + 1 < kin_hi) load_plane(kin_lo + 1, pf2);
-    for (int m = kin_lo; m < kin_hi; ++m) {
-        float *buf = lds + (m & 1) * plane;
+    auto step = [&](int m, float (&pa)[kBpLoads]) {
+        // keep the compiler from hoisting this step's LDS writes (and so
+        // the wait for its plane) into the previous step
+        __builtin_amdgcn_sched_barrier(0);
+        float *buf = lds + (m & 1) * kBpStride;
+        // slots past the window land in the buffer's padding (which holds
+        // only such filler), so every load is consumed by one LDS write
 #pragma unroll
-        for (int i = 0; i < kBpLoads; ++i) {
-            const int j = t + i * kBpThreads;
-            if (j < nload) buf[ldoff + j] = pf[i];
-        }
-#pragma unroll
-        for (int i = 0; i < kBpLoads; ++i) pf[i] = pf2[i];
-        if (m + 2 < kin_hi) load_plane(m + 2, pf2);
+        for (int i = 0; i < kBpLoads; ++i)
+            buf[(i < nslot ? ldoff + i * kBpThreads : kBpLoads * kBpThreads) + t] = pa[i];
+        load_plane(m + 2, pa, m + 2 < kin_hi);
         __syncthreads();
         const int k = m - R;  // plane completed this step
         const bool kin = k >= z0 && k < z1 && k >= R && k < nz - R;
-        float *orow = out + (long long)k * sz + (long long)y0 * nx;
+        const unsigned obase = (unsigned)(k * sz + (long long)y0 * nx) * 4u;
 #pragma unroll
-        for (int i = 0; i < kBpCells; ++i) {
-            if (!((own >> i) & 1u)) continue;
-            const int b = R * nx + t + i * kBpThreads;  // LDS index of the cell
+        for (int i = 0; i < NC; ++i) {
+            // cells past the frame read inside the LDS array and store nothing
+            const int b = R * nx + t + i * kBpThreads;
             float s = 0.0f;
 #pragma unroll
             for (int d = R; d >= 1; --d) s += buf[b - d];
@@ -200,8 +243,10 @@ __device__ __forceinline__ void bp_tile(const BatchProblem &q, int local, int sw
             const float cen = buf[b];
 #pragma unroll
             for (int j = 0; j < R; ++j) part[j][i] += cen;
-            if (kin && ((intr >> i) & 1u))
-                orow[t + i * kBpThreads] = finish<OP>(part[0][i], ring[0][i], q.kc);
+            const unsigned off = (kin && ((intr >> i) & 1u)) ? obase + (unsigned)(t + i * kBpThreads) * 4u
+                                                             : 0x80000000u;
+            __builtin_amdgcn_raw_buffer_store_b32(
+                __builtin_bit_cast(unsigned, finish<OP>(part[0][i], ring[0][i], q.kc)), rout, (int)off, 0, 0);
 #pragma unroll
             for (int j = 0; j + 1 < R; ++j) {
                 ring[j][i] = ring[j + 1][i];
@@ -210,20 +255,30 @@ __device__ __forceinline__ void bp_tile(const BatchProblem &q, int local, int sw
             ring[R - 1][i] = cen;
             part[R - 1][i] = s;
         }
+    };
+    // prologue in the loop's instruction pattern (dropped stores between the
+    // plane loads), so the wait counts at the loop head match the steady state
+    auto store_none = [&]() {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) __builtin_amdgcn_raw_buffer_store_b32(0u, rout, (int)0x80000000u, 0, 0);
+    };
+    load_plane(kin_lo, pf, true);
+    store_none();
+    load_plane(kin_lo + 1, pf2, kin_lo + 1 < kin_hi);
+    store_none();
+    // an odd step count runs one trailing step on no plane: it loads and
+    // stores nothing (k >= z1)
+    for (int m = kin_lo; m < kin_hi; m += 2) {
+        step(m, pf);
+        step(m + 1, pf2);
     }
     __syncthreads();  // the next tile rewrites the LDS planes
 }
 
-__device__ __forceinline__ void bp_item(const BatchProblem *__restrict__ P, int np, int item, int sweep,
-                                        float *lds)
+__device__ __forceinline__ void bp_item(const BatchProblem *__restrict__ P, const int *__restrict__ starts,
+                                        int np, int item, int sweep, float *lds)
 {
-    int lo = 0, hi = np - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (P[mid].item0 <= item) lo = mid;
-        else hi = mid - 1;
-    }
-    const BatchProblem &q = P[lo];
+    const BatchProblem &q = P[find_problem(starts, np, item)];
     if (sweep >= q.iters) return;
     const int local = item - q.item0;
     if (q.g.r == 1) {
@@ -235,21 +290,23 @@ __device__ __forceinline__ void bp_item(const BatchProblem *__restrict__ P, int 
     }
 }
 
-__global__ __launch_bounds__(kBpThreads) void k_bp_sweep(const BatchProblem *__restrict__ P, int np, int ntiles,
+__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_bp_sweep(const BatchProblem *__restrict__ P,
+                                                         const int *__restrict__ starts, int np, int ntiles,
                                                          int sweep)
 {
-    __shared__ float lds[2 * kBpLoads * kBpThreads];
+    __shared__ float lds[kBpLds];
     const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
-    if (tile < ntiles) bp_item(P, np, tile, sweep, lds);
+    if (tile < ntiles) bp_item(P, starts, np, tile, sweep, lds);
 }
 
-__global__ __launch_bounds__(kBpThreads) void k_bp_persistent(const BatchProblem *__restrict__ P, int np,
+__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_bp_persistent(const BatchProblem *__restrict__ P,
+                                                              const int *__restrict__ starts, int np,
                                                               int ntiles, int nsweeps)
 {
-    __shared__ float lds[2 * kBpLoads * kBpThreads];
+    __shared__ float lds[kBpLds];
     const int b0 = (int)xcd_remap(blockIdx.x, gridDim.x);
     for (int s = 0; s < nsweeps; ++s) {
-        for (int tile = b0; tile < ntiles; tile += (int)gridDim.x) bp_item(P, np, tile, s, lds);
+        for (int tile = b0; tile < ntiles; tile += (int)gridDim.x) bp_item(P, starts, np, tile, s, lds);
         if (s + 1 < nsweeps) cooperative_groups::this_grid().sync();
     }
 }
@@ -271,11 +328,13 @@ static int valid_ty(int r, int ty)
 }
 
 // The plane-chunk form takes a batch when every problem's frame of one row
-// plus 2R halo rows fits the LDS plane.
+// plus 2R halo rows fits the LDS plane and each buffer is addressable by a
+// 32-bit buffer offset.
 static bool bp_fits(const st_problem &p)
 {
-    return p.radius >= 1 && p.radius <= 2 && p.nx <= kBpCells * kBpThreads &&
-           (1 + 2 * p.radius) * p.nx <= kBpLoads * kBpThreads;
+    return p.radius >= 1 && p.radius <= 2 && p.nx <= bp_cells(p.radius) * kBpThreads &&
+           (1 + 2 * p.radius) * p.nx <= kBpLoads * kBpThreads &&
+           (long long)p.nx * p.ny * p.nz * 4 <= 0x7fffffffLL;
 }
 
 // Frame rows: the rule's coarsen_y when given, else as many rows as keep the
@@ -283,7 +342,7 @@ static bool bp_fits(const st_problem &p)
 // LDS plane.
 static int bp_rows(const st_problem &p, int want)
 {
-    const int lim = std::min(kBpCells * kBpThreads / p.nx, kBpLoads * kBpThreads / p.nx - 2 * p.radius);
+    const int lim = std::min(bp_cells(p.radius) * kBpThreads / p.nx, kBpLoads * kBpThreads / p.nx - 2 * p.radius);
     int f = want > 0 ? std::min(want, lim) : lim;
     if (f > p.ny) f = p.ny;
     return f < 1 ? 1 : f;
@@ -375,14 +434,20 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
     int nitems = 0, maxit = 0;
     st_status st = build_batch(probs, n, rules, nrules, table, nitems, maxit, plane_form);
     if (st != ST_OK) return st;
+    // device copy: the problem table, then each problem's first item
     void *ws = nullptr;
-    const size_t bytes = table.size() * sizeof(BatchProblem);
-    if ((st = workspace_slot(1, bytes, &ws)) != ST_OK) return st;
-    if ((st = hip_check(hipMemcpyAsync(ws, table.data(), bytes, hipMemcpyHostToDevice, s),
+    const size_t tbytes = (table.size() * sizeof(BatchProblem) + 255) & ~(size_t)255;
+    std::vector<unsigned char> host(tbytes + table.size() * sizeof(int), 0);
+    std::memcpy(host.data(), table.data(), table.size() * sizeof(BatchProblem));
+    for (size_t i = 0; i < table.size(); ++i)
+        std::memcpy(host.data() + tbytes + i * sizeof(int), &table[i].item0, sizeof(int));
+    if ((st = workspace_slot(1, host.size(), &ws)) != ST_OK) return st;
+    if ((st = hip_check(hipMemcpyAsync(ws, host.data(), host.size(), hipMemcpyHostToDevice, s),
                         "batch table copy")) != ST_OK)
         return st;
     if ((st = hip_check(hipStreamSynchronize(s), "batch table sync")) != ST_OK) return st;
     const BatchProblem *P = static_cast<const BatchProblem *>(ws);
+    const int *starts = reinterpret_cast<const int *>(static_cast<unsigned char *>(ws) + tbytes);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (elapsed_ms) {
         (void)hipEventCreate(&e0);
@@ -395,7 +460,8 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
         const void *fn = plane_form ? reinterpret_cast<const void *>(&k_bp_persistent)
                                     : reinterpret_cast<const void *>(&k_batch_persistent);
         int per_cu = 0, dev = 0, cus = 0;
-        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0);
+        const int threads = plane_form ? kBpThreads : 256;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0);
         if (e != hipSuccess) return hip_check(e, "occupancy");
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -403,20 +469,20 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
         if (grid > groups) grid = groups;
         if (grid < 1) grid = 1;
         int np = n, ni = nitems, ns = maxit;
-        void *args[] = {&P, &np, &ni, &ns};
+        void *args[] = {&P, &starts, &np, &ni, &ns};
         if (elapsed_ms) (void)hipEventRecord(e0, s);
-        e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(256), args, 0, s);
+        e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, s);
         if (e != hipSuccess) return hip_check(e, "batched cooperative launch");
     } else {
         // capture one launch per sweep, replay as one graph
-        const dim3 grid(groups), block(256);
+        const dim3 grid(groups), block(plane_form ? kBpThreads : 256);
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return hip_check(e, "begin capture");
         for (int sw = 0; sw < maxit; ++sw) {
-            if (plane_form) hipLaunchKernelGGL(k_bp_sweep, grid, block, 0, s, P, n, nitems, sw);
-            else hipLaunchKernelGGL(k_batch_sweep, grid, block, 0, s, P, n, nitems, sw);
+            if (plane_form) hipLaunchKernelGGL(k_bp_sweep, grid, block, 0, s, P, starts, n, nitems, sw);
+            else hipLaunchKernelGGL(k_batch_sweep, grid, block, 0, s, P, starts, n, nitems, sw);
         }
         e = hipStreamEndCapture(s, &graph);
         if (e != hipSuccess) return hip_check(e, "end capture");

@@ -58,7 +58,11 @@ template <int OP, int D, int RS, typename T>
 __global__ void k_smem(const T *__restrict__ in, T *__restrict__ out, Geom g, int cx, T kc)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    T *tile = reinterpret_cast<T *>(lds_raw);
+    // explicit LDS and global views: the per-neighbour "tile or global"
+    // choice then stays two loads and a value select instead of one flat
+    // load through a selected pointer
+    __attribute__((address_space(3))) T *tile = (__attribute__((address_space(3))) T *)lds_raw;
+    const __attribute__((address_space(1))) T *gin = (const __attribute__((address_space(1))) T *)in;
     const int R = RS ? RS : g.r;
     const int bx = blockDim.x, by = blockDim.y, bz = (D == 3) ? blockDim.z : 1;
     const int tx = threadIdx.x, ty = threadIdx.y, tz = (D == 3) ? threadIdx.z : 0;
@@ -71,7 +75,7 @@ __global__ void k_smem(const T *__restrict__ in, T *__restrict__ out, Geom g, in
     const int tbase = TX * (ty + by * tz);
     for (int s = 0; s < cx; ++s) {
         const int i = ib + s * bx;
-        if (row_in && i < g.nx) tile[tbase + tx + s * bx] = in[rowbase + i];
+        if (row_in && i < g.nx) tile[tbase + tx + s * bx] = gin[rowbase + i];
     }
     __syncthreads();
     if (!interior1(j, g.ny, R)) return;
@@ -85,18 +89,18 @@ __global__ void k_smem(const T *__restrict__ in, T *__restrict__ out, Geom g, in
         const long long idx = rowbase + i;
         T u = 0;
 #pragma unroll
-        for (int d = R; d >= 1; --d) u += (lx - d >= 0) ? tile[t - d] : in[idx - d];
+        for (int d = R; d >= 1; --d) u += (lx - d >= 0) ? tile[t - d] : gin[idx - d];
 #pragma unroll
-        for (int d = 1; d <= R; ++d) u += (lx + d < TX) ? tile[t + d] : in[idx + d];
+        for (int d = 1; d <= R; ++d) u += (lx + d < TX) ? tile[t + d] : gin[idx + d];
 #pragma unroll
-        for (int d = R; d >= 1; --d) u += (ty - d >= 0) ? tile[t - d * TX] : in[idx - d * g.sy];
+        for (int d = R; d >= 1; --d) u += (ty - d >= 0) ? tile[t - d * TX] : gin[idx - d * g.sy];
 #pragma unroll
-        for (int d = 1; d <= R; ++d) u += (ty + d < by) ? tile[t + d * TX] : in[idx + d * g.sy];
+        for (int d = 1; d <= R; ++d) u += (ty + d < by) ? tile[t + d * TX] : gin[idx + d * g.sy];
         if constexpr (D == 3) {
 #pragma unroll
-            for (int d = R; d >= 1; --d) u += (tz - d >= 0) ? tile[t - d * ZS] : in[idx - d * g.sz];
+            for (int d = R; d >= 1; --d) u += (tz - d >= 0) ? tile[t - d * ZS] : gin[idx - d * g.sz];
 #pragma unroll
-            for (int d = 1; d <= R; ++d) u += (tz + d < bz) ? tile[t + d * ZS] : in[idx + d * g.sz];
+            for (int d = 1; d <= R; ++d) u += (tz + d < bz) ? tile[t + d * ZS] : gin[idx + d * g.sz];
         }
         out[idx] = finish<OP>(u, tile[t], kc);
     }

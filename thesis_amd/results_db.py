@@ -299,7 +299,7 @@ def autotune_stencils(out: str, dimensions: Iterable[int] = (3,), dims: Optional
 # library's streaming copy kernel (profiles/traffic_512cubed_r1.md: FETCH_SIZE
 # counts half of the bytes read on this GPU, WRITE_SIZE all of them).
 PROFILE_PASSES = (("FETCH_SIZE",), ("WRITE_SIZE",),
-                  ("OccupancyPercent", "LDSBankConflict", "VALUBusy", "L2CacheHit"))
+                  ("OccupancyPercent", "LDSBankConflict", "VALUBusy", "TCC_HIT_sum", "TCC_MISS_sum"))
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 _HELPER_KERNELS = ("k_init", "k_shell_eq", "k_copy_f32", "k_fill16")
 
@@ -350,7 +350,8 @@ def rocprof_metrics(cmd: List[str], workdir: str, timeout_s: int = 120) -> Dict[
       achieved_occupancy      OccupancyPercent / 100 (fraction of 32 waves/CU)
       shared_ld_bank_conflict LDSBankConflict: % of GPU time LDS stalled on
                               bank conflicts (nvprof counted conflicts)
-      l2_tex_hit_rate         L2CacheHit (%)
+      l2_tex_hit_rate         100 * TCC_HIT / (TCC_HIT + TCC_MISS) (the
+                              derived L2CacheHit has no gfx950 definition)
       local_load_throughput, local_store_throughput
                               0 when the kernels use no scratch (spill) memory
     plus AMD-named values (valu_busy_pct, kernel_ms).  Counter values are
@@ -394,9 +395,9 @@ def rocprof_metrics(cmd: List[str], workdir: str, timeout_s: int = 120) -> Dict[
     lds = mean("LDSBankConflict")
     if lds is not None:
         out["shared_ld_bank_conflict"] = f"{lds:.4f}%"
-    l2 = mean("L2CacheHit")
-    if l2 is not None:
-        out["l2_tex_hit_rate"] = f"{l2:.2f}%"
+    hit, miss = mean("TCC_HIT_sum"), mean("TCC_MISS_sum")
+    if hit is not None and miss is not None and hit + miss > 0:
+        out["l2_tex_hit_rate"] = f"{100.0 * hit / (hit + miss):.2f}%"
     valu = mean("VALUBusy")
     if valu is not None:
         out["valu_busy_pct"] = f"{valu:.2f}%"
