@@ -91,30 +91,41 @@ def test_batched_full_modes_agree(gpu, oracle):
     assert np.array_equal(d1.numpy(stream).view(np.uint32), want.view(np.uint32))
 
 
-@pytest.mark.parametrize("shapes", [
+_PLANE_SHAPES = [(3, 3, 3, 1), (5, 40, 7, 2), (130, 3, 9, 1), (300, 50, 6, 1), (700, 9, 5, 2), (64, 64, 64, 2),
+                 (17, 200, 11, 1)]
+
+
+@pytest.mark.parametrize("shapes,op", [
     # plane-chunk form: minimal grids, one-row frames (nx = 700, R = 2), many
-    # frames, frames not dividing ny, chunks not dividing nz
-    [(3, 3, 3, 1), (5, 40, 7, 2), (130, 3, 9, 1), (300, 50, 6, 1), (700, 9, 5, 2), (64, 64, 64, 2),
-     (17, 200, 11, 1)],
+    # frames, frames not dividing ny, chunks not dividing nz.  THESIS problems
+    # start with buffer 2 zero (different shells: interior-only stores),
+    # JACOBI ones with equal shells (whole-row stores)
+    (_PLANE_SHAPES, 0),
+    (_PLANE_SHAPES, 1),
+    # mixed operators: the whole batch takes the march form
+    (_PLANE_SHAPES, None),
     # one problem too wide for the plane form: the whole batch takes the march form
-    [(1400, 5, 5, 1), (20, 18, 16, 2), (33, 31, 29, 1)],
-], ids=["plane", "march"])
-def test_batched_forms_edge_shapes(gpu, oracle, shapes):
+    ([(1400, 5, 5, 1), (20, 18, 16, 2), (33, 31, 29, 1)], None),
+], ids=["plane-thesis", "plane-jacobi", "mixed-ops", "march"])
+def test_batched_forms_edge_shapes(gpu, oracle, shapes, op):
     st = gpu
     stream = st.Stream()
     n = len(shapes)
     its = [3 + (i % 4) for i in range(n)]
-    ops = [i % 2 for i in range(n)]
+    ops = [i % 2 if op is None else op for i in range(n)]
     for mode in (0, 1):
-        print(f"[case] batched edge shapes mode {mode} {shapes}", flush=True)
+        print(f"[case] batched edge shapes mode {mode} op {op} {shapes}", flush=True)
         probs, host, bufs = _alloc(st, oracle, shapes, its, ops, 2000, stream)
         rules = [dict(radius=1, max_cells=10 ** 9, coarsen_y=0, zchunk=4),
                  dict(radius=2, max_cells=10 ** 9, coarsen_y=0, zchunk=0)]
         st.dispatch_batched(probs, rules, mode, stream=stream)
         stream.sync()
         for i, (q, (u1, u2)) in enumerate(zip(probs, host)):
-            want = oracle.run(u1.copy(), u2.copy(), 3, q["radius"], q["op"], q["iterations"])
-            got = (q["u2"] if q["iterations"] % 2 else q["u1"]).numpy(stream)
-            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (mode, i, shapes[i], q["op"])
+            a, b = u1.copy(), u2.copy()
+            oracle.run(a, b, 3, q["radius"], q["op"], q["iterations"])
+            # both buffers, shells included: the non-result buffer holds the
+            # previous sweep, and no shell cell may change
+            for got, want in ((q["u1"].numpy(stream), a), (q["u2"].numpy(stream), b)):
+                assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (mode, i, shapes[i], q["op"])
         for b in bufs:
             b.free()

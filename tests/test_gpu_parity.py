@@ -35,6 +35,18 @@ def run_gpu(st, params, u1_host, u2_host):
     return out, in_u2
 
 
+def run_gpu_both(st, params, u1_host, u2_host):
+    """Both buffers after the dispatch (result first)."""
+    d1 = st.DeviceArray.from_numpy(u1_host)
+    d2 = st.DeviceArray.from_numpy(u2_host)
+    in_u2, _ = st.dispatch_kernels(params, d1, d2)
+    st.synchronize()
+    a, b = d1.numpy(), d2.numpy()
+    d1.free()
+    d2.free()
+    return (b, a) if in_u2 else (a, b), in_u2
+
+
 def assert_bitwise(got, want, what=""):
     gb = np.ascontiguousarray(got).view(np.uint64 if got.dtype == np.float64 else np.uint32)
     wb = np.ascontiguousarray(want).view(np.uint64 if want.dtype == np.float64 else np.uint32)
@@ -350,9 +362,15 @@ def test_coop_stream_form(gpu, oracle, shape, knobs, op, init):
                                   op=op, variant="coop", temporal_form=3, **kw)
         cfg = st.launch_config(p)
         assert cfg["launches"] == 1
-        got, in_u2 = run_gpu(st, p, u1, u2)
+        (got, other), in_u2 = run_gpu_both(st, p, u1, u2)
         assert in_u2 == bool(its % 2)
         assert_bitwise(got, want, f"stream form {shape} {knobs} op={op} {init} its={its}")
+        # the other buffer's interior is an intermediate state; its shell
+        # (stored whole with equal shells) must hold its original bits
+        shell = np.ones(u1.shape, bool)
+        shell[1:-1, 1:-1, 1:-1] = False
+        orig = u1 if in_u2 else u2
+        assert_bitwise(other[shell], orig[shell], f"stream form shell {shape} {knobs} op={op} {init} its={its}")
 
 
 @pytest.mark.parametrize("face", ["x0", "x1", "y0", "y1", "z0", "z1", "edge", "corner"])
@@ -418,14 +436,19 @@ def test_smem_register_rows_form(gpu, oracle, shape, dt, radii, knobs, op):
         kw.update(knobs)
         if kw["heuristic"] == 0:
             kw["block_x"] = nx
-        for its in (1, 3):
-            want = oracle.run(u1.copy(), u2.copy(), 3, r, op, its).copy()
+        for its in (1, 3, 4):
+            # op 1 starts with equal shells (whole-row stores), op 0 with
+            # buffer 2 zero; both buffers are checked, shells included
+            a, b = u1.copy(), u2.copy()
+            want = oracle.run(a, b, 3, r, op, its)
+            other = b if want is a else a
             p = st.StencilParams.make(nx=nx, ny=ny, nz=nz, dimensions=3, radius=r, iterations=its, op=op,
                                       dtype=st.F64 if dt == "f64" else st.F32, variant="smem_register",
                                       temporal_form=2, **kw)
-            got, in_u2 = run_gpu(st, p, u1, u2)
+            (got, got_other), in_u2 = run_gpu_both(st, p, u1, u2)
             assert in_u2 == bool(its % 2)
             assert_bitwise(got, want, f"rows form {shape} {dt} r={r} op={op} its={its}")
+            assert_bitwise(got_other, other, f"rows form other buffer {shape} {dt} r={r} op={op} its={its}")
 
 
 def test_smem_register_rows_form_rejects_unsupported(gpu):

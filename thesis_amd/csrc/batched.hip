@@ -8,8 +8,10 @@
 //   * all problems' work is one flat list of tiles ordered by problem; a
 //     tile finds its problem by binary search over the list offsets.  The
 //     tiles are workgroup tiles of the plane-chunk form (below) when every
-//     problem fits it, else wave tiles of the register-queue march (64
-//     columns x TY rows x zchunk planes);
+//     problem fits it and all share one operator, else wave tiles of the
+//     register-queue march (64 columns x TY rows x zchunk planes);
+//   * the plane form first compares each problem's two shells; with equal
+//     shells it stores whole rows (shell cells with their own value);
 //   * ST_BATCH_GRAPH captures one launch per sweep (the whole batch in each)
 //     into a hipGraph and replays it as a single graph launch;
 //   * ST_BATCH_PERSISTENT runs every sweep inside one cooperative kernel
@@ -138,7 +140,7 @@ constexpr int kBpStride = kBpLoads * kBpThreads + kBpThreads;
 constexpr int kBpLds = kBpStride + 4800;
 
 template <int OP, int R>
-__device__ __forceinline__ void bp_tile(const BatchProblem &q, int local, int sweep, float *lds)
+__device__ __forceinline__ void bp_tile(const BatchProblem &q, int local, int sweep, bool eq, float *lds)
 {
     constexpr int NC = bp_cells(R);
     const int t = threadIdx.x;
@@ -157,8 +159,12 @@ __device__ __forceinline__ void bp_tile(const BatchProblem &q, int local, int sw
     const int nslot = (nload + kBpThreads - 1) / kBpThreads;  // load slots in use
     const int ldoff = (rlo - (y0 - R)) * nx;  // LDS offset of row rlo
 
-    // owned cells: interior mask (x, y, plus the z test per plane)
-    unsigned intr = 0;
+    // owned cells: interior mask (x, y, plus the z test per plane), and the
+    // mask of stored cells: the interior, or with equal shells whole rows of
+    // the interior y range (a shell cell is written with the value it has,
+    // so stores cover whole cache lines; partially written lines cost a
+    // read-modify-write in HBM)
+    unsigned intr = 0, put = 0;
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         const int c = t + i * kBpThreads;
@@ -166,6 +172,7 @@ __device__ __forceinline__ void bp_tile(const BatchProblem &q, int local, int sw
             const int r = c / nx, x = c - r * nx;
             const int y = y0 + r;
             if (x >= R && x < nx - R && y >= R && y < ny - R) intr |= 1u << i;
+            if (y >= R && y < ny - R && (eq || (x >= R && x < nx - R))) put |= 1u << i;
         }
     }
 
@@ -243,10 +250,10 @@ __device__ __forceinline__ void bp_tile(const BatchProblem &q, int local, int sw
             const float cen = buf[b];
 #pragma unroll
             for (int j = 0; j < R; ++j) part[j][i] += cen;
-            const unsigned off = (kin && ((intr >> i) & 1u)) ? obase + (unsigned)(t + i * kBpThreads) * 4u
-                                                             : 0x80000000u;
-            __builtin_amdgcn_raw_buffer_store_b32(
-                __builtin_bit_cast(unsigned, finish<OP>(part[0][i], ring[0][i], q.kc)), rout, (int)off, 0, 0);
+            const unsigned off = (kin && ((put >> i) & 1u)) ? obase + (unsigned)(t + i * kBpThreads) * 4u
+                                                            : 0x80000000u;
+            const float res = ((intr >> i) & 1u) ? finish<OP>(part[0][i], ring[0][i], q.kc) : ring[0][i];
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, res), rout, (int)off, 0, 0);
 #pragma unroll
             for (int j = 0; j + 1 < R; ++j) {
                 ring[j][i] = ring[j + 1][i];
@@ -275,38 +282,66 @@ __device__ __forceinline__ void bp_tile(const BatchProblem &q, int local, int sw
     __syncthreads();  // the next tile rewrites the LDS planes
 }
 
-__device__ __forceinline__ void bp_item(const BatchProblem *__restrict__ P, const int *__restrict__ starts,
-                                        int np, int item, int sweep, float *lds)
+// Shell comparison of every problem: flags[p] is set to 1 by k_bp_flags,
+// then cleared by any block of k_bp_shell_eq (grid: problems x parts) that
+// finds a cell of problem p's R-thick shell whose bits differ between u1 and
+// u2.
+__global__ __launch_bounds__(256) void k_bp_flags(int *__restrict__ flags, int np)
 {
-    const BatchProblem &q = P[find_problem(starts, np, item)];
-    if (sweep >= q.iters) return;
-    const int local = item - q.item0;
-    if (q.g.r == 1) {
-        if (q.op == OP_THESIS) bp_tile<OP_THESIS, 1>(q, local, sweep, lds);
-        else bp_tile<OP_JACOBI, 1>(q, local, sweep, lds);
-    } else {
-        if (q.op == OP_THESIS) bp_tile<OP_THESIS, 2>(q, local, sweep, lds);
-        else bp_tile<OP_JACOBI, 2>(q, local, sweep, lds);
-    }
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < np) flags[i] = 1;
 }
 
-__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_bp_sweep(const BatchProblem *__restrict__ P,
-                                                         const int *__restrict__ starts, int np, int ntiles,
-                                                         int sweep)
+__global__ __launch_bounds__(256) void k_bp_shell_eq(const BatchProblem *__restrict__ P, int *__restrict__ flags)
+{
+    const BatchProblem &q = P[blockIdx.x];
+    const int nx = q.g.nx, ny = q.g.ny, nz = q.g.nz, R = q.g.r;
+    const unsigned *a = reinterpret_cast<const unsigned *>(q.u1);
+    const unsigned *b = reinterpret_cast<const unsigned *>(q.u2);
+    const long long n1 = shell_full_cells(nx, ny, nz, R), n = shell_cells(nx, ny, nz, R);
+    int ok = 1;
+    for (long long i = blockIdx.y * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.y) {
+        const long long idx = shell_cell(i, n1, nx, ny, nz, R);
+        ok &= a[idx] == b[idx];
+    }
+    ok = __syncthreads_and(ok);
+    if (threadIdx.x == 0 && !ok) flags[blockIdx.x] = 0;
+}
+
+// The plane form is instantiated per operator (a batch takes it when all its
+// problems share one), so each kernel inlines only the two radius bodies.
+template <int OP>
+__device__ __forceinline__ void bp_item(const BatchProblem *__restrict__ P, const int *__restrict__ starts,
+                                        const int *__restrict__ eqf, int np, int item, int sweep, float *lds)
+{
+    const int pi = find_problem(starts, np, item);
+    const BatchProblem &q = P[pi];
+    if (sweep >= q.iters) return;
+    const int local = item - q.item0;
+    const bool eq = eqf[pi] != 0;
+    if (q.g.r == 1) bp_tile<OP, 1>(q, local, sweep, eq, lds);
+    else bp_tile<OP, 2>(q, local, sweep, eq, lds);
+}
+
+template <int OP>
+__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(OP == OP_THESIS ? 4 : 6))) void k_bp_sweep(
+    const BatchProblem *__restrict__ P, const int *__restrict__ starts, const int *__restrict__ eqf, int np,
+    int ntiles, int sweep)
 {
     __shared__ float lds[kBpLds];
     const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
-    if (tile < ntiles) bp_item(P, starts, np, tile, sweep, lds);
+    if (tile < ntiles) bp_item<OP>(P, starts, eqf, np, tile, sweep, lds);
 }
 
-__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_bp_persistent(const BatchProblem *__restrict__ P,
-                                                              const int *__restrict__ starts, int np,
-                                                              int ntiles, int nsweeps)
+template <int OP>
+__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(OP == OP_THESIS ? 4 : 6))) void k_bp_persistent(
+    const BatchProblem *__restrict__ P, const int *__restrict__ starts, const int *__restrict__ eqf, int np,
+    int ntiles, int nsweeps)
 {
     __shared__ float lds[kBpLds];
     const int b0 = (int)xcd_remap(blockIdx.x, gridDim.x);
     for (int s = 0; s < nsweeps; ++s) {
-        for (int tile = b0; tile < ntiles; tile += (int)gridDim.x) bp_item(P, starts, np, tile, s, lds);
+        for (int tile = b0; tile < ntiles; tile += (int)gridDim.x) bp_item<OP>(P, starts, eqf, np, tile, s, lds);
         if (s + 1 < nsweeps) cooperative_groups::this_grid().sync();
     }
 }
@@ -429,25 +464,43 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
         return ST_OK;
     }
     bool plane_form = true;
-    for (int i = 0; i < n; ++i) plane_form = plane_form && bp_fits(probs[i]);
+    for (int i = 0; i < n; ++i) plane_form = plane_form && bp_fits(probs[i]) && probs[i].op == probs[0].op;
+    const bool thesis = probs[0].op == ST_OP_THESIS;
     std::vector<BatchProblem> table;
     int nitems = 0, maxit = 0;
     st_status st = build_batch(probs, n, rules, nrules, table, nitems, maxit, plane_form);
     if (st != ST_OK) return st;
-    // device copy: the problem table, then each problem's first item
+    // device copy: the problem table, then each problem's first item, then
+    // room for the shell-comparison flags
     void *ws = nullptr;
     const size_t tbytes = (table.size() * sizeof(BatchProblem) + 255) & ~(size_t)255;
-    std::vector<unsigned char> host(tbytes + table.size() * sizeof(int), 0);
+    const size_t sbytes = (table.size() * sizeof(int) + 255) & ~(size_t)255;
+    const size_t fbytes = table.size() * sizeof(int);
+    std::vector<unsigned char> host(tbytes + sbytes, 0);
     std::memcpy(host.data(), table.data(), table.size() * sizeof(BatchProblem));
     for (size_t i = 0; i < table.size(); ++i)
         std::memcpy(host.data() + tbytes + i * sizeof(int), &table[i].item0, sizeof(int));
-    if ((st = workspace_slot(1, host.size(), &ws)) != ST_OK) return st;
+    if ((st = workspace_slot(1, host.size() + fbytes, &ws)) != ST_OK) return st;
     if ((st = hip_check(hipMemcpyAsync(ws, host.data(), host.size(), hipMemcpyHostToDevice, s),
                         "batch table copy")) != ST_OK)
         return st;
     if ((st = hip_check(hipStreamSynchronize(s), "batch table sync")) != ST_OK) return st;
     const BatchProblem *P = static_cast<const BatchProblem *>(ws);
     const int *starts = reinterpret_cast<const int *>(static_cast<unsigned char *>(ws) + tbytes);
+    int *eqf = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + tbytes + sbytes);
+    const int *ceqf = eqf;
+    // plane form: compare the problems' shells first (on the stream, inside
+    // the timed region and the graph)
+    // parts per problem: about 8192 shell cells per block of the largest shell
+    long long max_shell = 1;
+    for (int i = 0; i < n; ++i)
+        max_shell = std::max(max_shell, shell_cells(probs[i].nx, probs[i].ny, probs[i].nz, probs[i].radius));
+    const int parts = (int)std::min<long long>(256, (max_shell + 8191) / 8192);
+    auto shell_check = [&]() {
+        if (!plane_form) return;
+        hipLaunchKernelGGL(k_bp_flags, dim3((n + 255) / 256), dim3(256), 0, s, eqf, n);
+        hipLaunchKernelGGL(k_bp_shell_eq, dim3(n, parts), dim3(256), 0, s, P, eqf);
+    };
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (elapsed_ms) {
         (void)hipEventCreate(&e0);
@@ -457,8 +510,9 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
     // tiles of the plane-chunk form
     const int groups = plane_form ? nitems : (nitems + 3) / 4;
     if (mode == ST_BATCH_PERSISTENT) {
-        const void *fn = plane_form ? reinterpret_cast<const void *>(&k_bp_persistent)
-                                    : reinterpret_cast<const void *>(&k_batch_persistent);
+        const void *fn = !plane_form ? reinterpret_cast<const void *>(&k_batch_persistent)
+                         : thesis    ? reinterpret_cast<const void *>(&k_bp_persistent<OP_THESIS>)
+                                     : reinterpret_cast<const void *>(&k_bp_persistent<OP_JACOBI>);
         int per_cu = 0, dev = 0, cus = 0;
         const int threads = plane_form ? kBpThreads : 256;
         hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0);
@@ -469,9 +523,11 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
         if (grid > groups) grid = groups;
         if (grid < 1) grid = 1;
         int np = n, ni = nitems, ns = maxit;
-        void *args[] = {&P, &starts, &np, &ni, &ns};
+        void *args_bp[] = {&P, &starts, &ceqf, &np, &ni, &ns};
+        void *args_march[] = {&P, &starts, &np, &ni, &ns};
         if (elapsed_ms) (void)hipEventRecord(e0, s);
-        e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, s);
+        shell_check();
+        e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), plane_form ? args_bp : args_march, 0, s);
         if (e != hipSuccess) return hip_check(e, "batched cooperative launch");
     } else {
         // capture one launch per sweep, replay as one graph
@@ -480,8 +536,12 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
         hipGraphExec_t exec = nullptr;
         hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return hip_check(e, "begin capture");
+        shell_check();
         for (int sw = 0; sw < maxit; ++sw) {
-            if (plane_form) hipLaunchKernelGGL(k_bp_sweep, grid, block, 0, s, P, starts, n, nitems, sw);
+            if (plane_form && thesis)
+                hipLaunchKernelGGL(k_bp_sweep<OP_THESIS>, grid, block, 0, s, P, starts, ceqf, n, nitems, sw);
+            else if (plane_form)
+                hipLaunchKernelGGL(k_bp_sweep<OP_JACOBI>, grid, block, 0, s, P, starts, ceqf, n, nitems, sw);
             else hipLaunchKernelGGL(k_batch_sweep, grid, block, 0, s, P, starts, n, nitems, sw);
         }
         e = hipStreamEndCapture(s, &graph);

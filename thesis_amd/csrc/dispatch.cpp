@@ -156,6 +156,7 @@ static st_status resolve(const st_params *p, Cfg &c, bool mixed)
     c.kind = 0;
     c.wx = c.wy = 0;
     c.hs = 1;
+    c.shell_eq = nullptr;
     c.zchunk = 0;
     c.lds = 0;
     c.gx = c.gy = c.gz = 1;
@@ -285,6 +286,10 @@ st_status stencil_dispatch(const st_params *p, void *d_u1, void *d_u2, void *str
         if (c.variant == ST_V_COOP) {
             s = run_coop(c, d_u1, d_u2, hs, &in_u2);
         } else {
+            // the row form stores whole rows when both shells are equal
+            // (a shell cell then holds the same bits at every sweep)
+            if (c.variant == ST_V_SMEM_REGISTER && c.kind == 4 && c.iters > 1)
+                s = rows_shell_check(c, d_u1, d_u2, hs, &c.shell_eq);
             void *a = d_u1, *b = d_u2;
             for (int i = 0; i < c.iters && s == ST_OK; ++i) {
                 s = sweep_once(c, a, b, hs);

@@ -67,6 +67,50 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n)
     return (b & 7u) * (n8 >> 3) + (b >> 3);
 }
 
+// The R-thick shell of an nx x ny x nz grid (every cell that is not
+// interior) as a list: first the full shell rows (the 2R outer planes, then
+// the 2R outer rows of each inner plane) cell by cell along x, then the 2R
+// end cells (x < R, x >= nx - R) of each inner row.  Consecutive list
+// positions are consecutive memory in the first part, so lanes read it
+// coalesced.
+__host__ __device__ inline long long shell_full_cells(int nx, int ny, int nz, int R)
+{
+    return (2LL * R * ny + (long long)(nz - 2 * R) * 2 * R) * nx;
+}
+__host__ __device__ inline long long shell_cells(int nx, int ny, int nz, int R)
+{
+    const long long inner = (long long)(nz - 2 * R) * (ny - 2 * R);
+    return shell_full_cells(nx, ny, nz, R) + (inner > 0 ? inner * 2 * R : 0);
+}
+// Linear index (x fastest) of list position i < shell_cells(...); n1 =
+// shell_full_cells(...).
+__device__ __forceinline__ long long shell_cell(long long i, long long n1, int nx, int ny, int nz, int R)
+{
+    int x, y, z;
+    if (i < n1) {
+        const long long row = i / nx;
+        x = (int)(i - row * nx);
+        if (row < 2LL * R * ny) {  // an outer plane
+            const int pl = (int)(row / ny);
+            z = pl < R ? pl : nz - 2 * R + pl;
+            y = (int)(row - (long long)pl * ny);
+        } else {                   // an outer row of an inner plane
+            const long long r2 = row - 2LL * R * ny;
+            z = R + (int)(r2 / (2 * R));
+            const int m = (int)(r2 % (2 * R));
+            y = m < R ? m : ny - 2 * R + m;
+        }
+    } else {
+        const long long j = i - n1;
+        const long long row = j / (2 * R);
+        const int m = (int)(j - row * 2 * R);
+        z = R + (int)(row / (ny - 2 * R));
+        y = R + (int)(row % (ny - 2 * R));
+        x = m < R ? m : nx - 2 * R + m;
+    }
+    return (long long)z * nx * ny + (long long)y * nx + x;
+}
+
 constexpr int floor_div(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
 
 }  // namespace st

@@ -30,6 +30,8 @@ struct Cfg {
     int kind;            // COOP form resolved
     int wx, wy;          // COOP register form: waves across x and y
     int hs;              // SMEM_REGISTER row form: halo strips per wave
+    const int *shell_eq; // device flag: both buffers' shells hold the same bits
+                         // (row form stores whole rows then); null: unknown
 };
 
 // Record a detail string for stencil_last_error() and return `s`.
@@ -65,8 +67,12 @@ st_status resolve_rows(Cfg &c);
 st_status launch_rows(const Cfg &c, const void *in, void *out, hipStream_t s);
 // 16-bit storage modes on the geometry of `c` (FP32 arithmetic, JACOBI
 // weight c.weight): defect d = round16(w*sum(x) - x), else the correction
-// e' = round16(widen(d) + w*sum(widen(e)))
+// e' = round16(widen(d) + w*sum(widen(e))); the output's shell must be zero
+// (whole edge quads are stored, with zero in the shell cells)
 bool rows_mixed_eligible(const Cfg &c);
+// Compare the R-thick shells of u1 and u2 on the stream; *flag receives a
+// device pointer to an int that is 1 when they hold the same bits.
+st_status rows_shell_check(const Cfg &c, const void *u1, const void *u2, hipStream_t s, const int **flag);
 st_status launch_rows_mixed(const Cfg &c, int defect, bool f16, const void *in, void *out, const uint16_t *d,
                             hipStream_t s);
 

@@ -43,6 +43,8 @@ struct RowsArgs {
     int wy;         // core rows per frame
     int tiles_y, zchunk, ntiles;
     int nt;         // non-temporal stores
+    const int *eq;  // sweep mode: 1 when both buffers' shells are equal
+                    // (edge quads then store whole, shell cells unchanged)
 };
 
 template <typename T> struct Quad;
@@ -118,6 +120,7 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
     const int QW = wx * H;                     // quads per LDS row
     const int lrows = wy + 2 * R;              // LDS rows per buffer: R halo, wy core, R halo
     const bool first = lane == 0, last = lane == 63;
+    const bool eqs = MODE == RM_SWEEP && a.eq != nullptr && *a.eq != 0;
 
     // own quads: columns 256*(wxp*H + h) + 4*lane
     int xo[H];
@@ -345,6 +348,15 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
                             for (int j = 0; j < 4; ++j) q[j] = res[4 * h + j];
                             if (a.nt) __builtin_nontemporal_store(q, reinterpret_cast<QV *>(row + xo[h]));
                             else *reinterpret_cast<QV *>(row + xo[h]) = q;
+                        } else if (act[h] && eqs) {
+                            // equal shells: the whole quad, shell cells with
+                            // the value they hold (full cache lines)
+                            QV q;
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+                                q[j] = interior1(xo[h] + j, a.nx, R) ? res[4 * h + j] : ring[0][4 * h + j];
+                            if (a.nt) __builtin_nontemporal_store(q, reinterpret_cast<QV *>(row + xo[h]));
+                            else *reinterpret_cast<QV *>(row + xo[h]) = q;
                         } else if (act[h]) {
 #pragma unroll
                             for (int j = 0; j < 4; ++j)
@@ -369,9 +381,14 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
                         if (intq[h]) {
                             *reinterpret_cast<U4 *>(row + xo[h]) = res[h];
                         } else if (act[h]) {
+                            // the 16-bit outputs' shells are zero (the
+                            // mixed driver fills them and never writes them),
+                            // so edge quads store whole with zero there
+                            U4 q;
 #pragma unroll
                             for (int j = 0; j < 4; ++j)
-                                if (interior1(xo[h] + j, a.nx, R)) row[xo[h] + j] = res[h][j];
+                                q[j] = interior1(xo[h] + j, a.nx, R) ? res[h][j] : (uint16_t)0;
+                            *reinterpret_cast<U4 *>(row + xo[h]) = q;
                         }
                     }
                 }
@@ -565,6 +582,7 @@ static st_status rows_launch(const Cfg &c, const void *fn, const void *in, void 
     a.zchunk = c.zchunk;
     a.ntiles = a.tiles_y * ((c.nz + c.zchunk - 1) / c.zchunk);
     a.nt = 1;
+    a.eq = c.shell_eq;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds);
     if (e != hipSuccess) return hip_check(e, "hipFuncSetAttribute(rows LDS)");
     const int nthreads = 64 * c.wx * c.wy;
@@ -589,6 +607,49 @@ st_status launch_rows(const Cfg &c, const void *in, void *out, hipStream_t s)
     return rows_launch(c, fn, in, out, nullptr, s);
 }
 
+// Shell comparison of one grid: k_flag_one sets the flag, any block that
+// finds a shell cell whose bits differ clears it.
+__global__ __launch_bounds__(64) void k_flag_one(int *__restrict__ flag)
+{
+    if (threadIdx.x == 0) *flag = 1;
+}
+
+template <typename W>
+__global__ __launch_bounds__(256) void k_grid_shell_eq(const W *__restrict__ u1, const W *__restrict__ u2, int nx,
+                                                       int ny, int nz, int R, int *__restrict__ flag)
+{
+    const long long n1 = shell_full_cells(nx, ny, nz, R), n = shell_cells(nx, ny, nz, R);
+    int ok = 1;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) {
+        const long long idx = shell_cell(i, n1, nx, ny, nz, R);
+        ok &= u1[idx] == u2[idx];
+    }
+    ok = __syncthreads_and(ok);
+    if (threadIdx.x == 0 && !ok) *flag = 0;
+}
+
+st_status rows_shell_check(const Cfg &c, const void *u1, const void *u2, hipStream_t s, const int **flag)
+{
+    void *ws = nullptr;
+    st_status st = workspace_slot(2, sizeof(int), &ws);
+    if (st != ST_OK) return st;
+    int *f = static_cast<int *>(ws);
+    const long long n = shell_cells(c.nx, c.ny, c.nz, c.r);
+    const int blocks = (int)std::max<long long>(1, std::min<long long>(2048, (n + 2047) / 2048));
+    hipLaunchKernelGGL(k_flag_one, dim3(1), dim3(64), 0, s, f);
+    if (c.dtype == ST_F64)
+        hipLaunchKernelGGL(k_grid_shell_eq<unsigned long long>, dim3(blocks), dim3(256), 0, s,
+                           static_cast<const unsigned long long *>(u1), static_cast<const unsigned long long *>(u2),
+                           c.nx, c.ny, c.nz, c.r, f);
+    else
+        hipLaunchKernelGGL(k_grid_shell_eq<unsigned>, dim3(blocks), dim3(256), 0, s, static_cast<const unsigned *>(u1),
+                           static_cast<const unsigned *>(u2), c.nx, c.ny, c.nz, c.r, f);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_check(e, "shell comparison launch");
+    *flag = f;
+    return ST_OK;
+}
+
 // The 16-bit modes resolve their own layout (their register need differs
 // from the sweep's by the defect row).
 bool rows_mixed_eligible(const Cfg &c)
@@ -602,6 +663,7 @@ bool rows_mixed_eligible(const Cfg &c)
            rows_resolve_mode(g, RM_DEFECT, c.dtype == ST_F16) == ST_OK;
 }
 
+// `out` must have a zero shell (edge quads write zeros there).
 st_status launch_rows_mixed(const Cfg &c, int defect, bool f16, const void *in, void *out, const uint16_t *d,
                             hipStream_t s)
 {

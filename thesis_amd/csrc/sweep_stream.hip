@@ -111,6 +111,8 @@ struct SmLane {
     bool act[CX / 4];  // quad h lies inside the row
     bool lo[CX / 4];   // quad h starts at column 0
     bool hi[CX / 4];   // quad h ends at column nx-1
+    bool whole[CX / 4];  // quad h is stored as one vector (interior, or any
+                         // quad inside the row when both shells are equal)
 };
 
 // Per-tile, per-wave constants.
@@ -257,6 +259,10 @@ __device__ __forceinline__ void sm_finish(SmPlane<CX, RPW> &out, const SmPlane<C
     }
 }
 
+// Store the interior rows of plane k.  With equal shells (SmLane::whole)
+// the x shell cells, which hold their unchanged value, are stored too, so
+// every row is written as whole cache lines (a partly written line costs
+// HBM a read-modify-write); otherwise they are skipped.
 template <int CX, int RPW>
 __device__ __forceinline__ void sm_store(float *__restrict__ Y, const SmPlane<CX, RPW> &v,
                                          const SmArgs &a, const SmTile<RPW> &t, const SmLane<CX> &l,
@@ -271,7 +277,7 @@ __device__ __forceinline__ void sm_store(float *__restrict__ Y, const SmPlane<CX
         float *row = base + t.rowoff[r];
 #pragma unroll
         for (int h = 0; h < CX / 4; ++h) {
-            if (l.act[h] && !l.lo[h] && !l.hi[h]) {
+            if (l.whole[h]) {
                 V4 q = {v.v[r][4 * h], v.v[r][4 * h + 1], v.v[r][4 * h + 2], v.v[r][4 * h + 3]};
                 __builtin_nontemporal_store(q, reinterpret_cast<V4 *>(row + l.xo[h]));
             } else if (l.act[h]) {
@@ -287,7 +293,7 @@ __device__ __forceinline__ void sm_store(float *__restrict__ Y, const SmPlane<CX
 template <int OP, int CX, int RPW, int T>
 __device__ __forceinline__ void sm_pass(const float *__restrict__ X, float *__restrict__ Y,
                                         const SmArgs &a, float kc, float4 *lds, unsigned bid,
-                                        unsigned nblk)
+                                        unsigned nblk, bool same)
 {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -300,6 +306,7 @@ __device__ __forceinline__ void sm_pass(const float *__restrict__ X, float *__re
         l.xo[h] = l.act[h] ? x : a.nx - 4;  // quads past the row reload its last cells
         l.lo[h] = x == 0;
         l.hi[h] = x + 4 == a.nx;
+        l.whole[h] = l.act[h] && (same || (!l.lo[h] && !l.hi[h]));
     }
     float4 *reg0 = lds;
     float4 *reg1 = lds + 2 * wy * (CX / 4) * 64;
@@ -377,8 +384,8 @@ __global__ __launch_bounds__((sm_max_threads<CX, RPW>())) void k_stream(float *u
     for (int p = 0; p < npass; ++p) {
         const float *X = (p & 1) ? u2 : u1;
         float *Y = (p & 1) ? u1 : u2;
-        if (same && p < a.n2) sm_pass<OP, CX, RPW, 2>(X, Y, a, kc, lds, bid, nblk);
-        else sm_pass<OP, CX, RPW, 1>(X, Y, a, kc, lds, bid, nblk);
+        if (same && p < a.n2) sm_pass<OP, CX, RPW, 2>(X, Y, a, kc, lds, bid, nblk, same);
+        else sm_pass<OP, CX, RPW, 1>(X, Y, a, kc, lds, bid, nblk, same);
         if (p + 1 < npass) cooperative_groups::this_grid().sync();
     }
 }
