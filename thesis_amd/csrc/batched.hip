@@ -298,9 +298,10 @@ __global__ __launch_bounds__(256) void k_bp_shell_eq(const BatchProblem *__restr
     const int nx = q.g.nx, ny = q.g.ny, nz = q.g.nz, R = q.g.r;
     const unsigned *a = reinterpret_cast<const unsigned *>(q.u1);
     const unsigned *b = reinterpret_cast<const unsigned *>(q.u2);
-    const long long n1 = shell_full_cells(nx, ny, nz, R), n = shell_cells(nx, ny, nz, R);
+    // (a problem of the plane form holds under 2^29 cells)
+    const unsigned n1 = (unsigned)shell_full_cells(nx, ny, nz, R), n = (unsigned)shell_cells(nx, ny, nz, R);
     int ok = 1;
-    for (long long i = blockIdx.y * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.y) {
+    for (unsigned i = blockIdx.y * 256u + threadIdx.x; i < n; i += 256u * gridDim.y) {
         const long long idx = shell_cell(i, n1, nx, ny, nz, R);
         ok &= a[idx] == b[idx];
     }

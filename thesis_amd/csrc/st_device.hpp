@@ -82,31 +82,34 @@ __host__ __device__ inline long long shell_cells(int nx, int ny, int nz, int R)
     const long long inner = (long long)(nz - 2 * R) * (ny - 2 * R);
     return shell_full_cells(nx, ny, nz, R) + (inner > 0 ? inner * 2 * R : 0);
 }
-// Linear index (x fastest) of list position i < shell_cells(...); n1 =
+// Linear index (x fastest) of list position i < shell_cells(...) (which the
+// callers keep below 2^31, so the list arithmetic is 32-bit); n1 =
 // shell_full_cells(...).
-__device__ __forceinline__ long long shell_cell(long long i, long long n1, int nx, int ny, int nz, int R)
+__device__ __forceinline__ long long shell_cell(unsigned i, unsigned n1, int nx, int ny, int nz, int R)
 {
-    int x, y, z;
+    unsigned x, y, z;
+    const unsigned R2 = 2u * R;
     if (i < n1) {
-        const long long row = i / nx;
-        x = (int)(i - row * nx);
-        if (row < 2LL * R * ny) {  // an outer plane
-            const int pl = (int)(row / ny);
-            z = pl < R ? pl : nz - 2 * R + pl;
-            y = (int)(row - (long long)pl * ny);
+        const unsigned row = i / nx;
+        x = i - row * nx;
+        if (row < R2 * ny) {       // an outer plane
+            const unsigned pl = row / ny;
+            z = pl < (unsigned)R ? pl : nz - R2 + pl;
+            y = row - pl * ny;
         } else {                   // an outer row of an inner plane
-            const long long r2 = row - 2LL * R * ny;
-            z = R + (int)(r2 / (2 * R));
-            const int m = (int)(r2 % (2 * R));
-            y = m < R ? m : ny - 2 * R + m;
+            const unsigned r2 = row - R2 * ny;
+            z = R + r2 / R2;
+            const unsigned m = r2 % R2;
+            y = m < (unsigned)R ? m : ny - R2 + m;
         }
     } else {
-        const long long j = i - n1;
-        const long long row = j / (2 * R);
-        const int m = (int)(j - row * 2 * R);
-        z = R + (int)(row / (ny - 2 * R));
-        y = R + (int)(row % (ny - 2 * R));
-        x = m < R ? m : nx - 2 * R + m;
+        const unsigned j = i - n1;
+        const unsigned row = j / R2;
+        const unsigned m = j - row * R2;
+        const unsigned in = ny - R2;
+        z = R + row / in;
+        y = R + row % in;
+        x = m < (unsigned)R ? m : nx - R2 + m;
     }
     return (long long)z * nx * ny + (long long)y * nx + x;
 }
