@@ -19,14 +19,19 @@
  *       idx = i + j*nx + k*nx*ny          (src/stencil_kernel_base.cu:14)
  *   - Device pointers are plain HIP device addresses; `stream` is a
  *     hipStream_t passed as void* (NULL = the null stream).
- *   - Only interior cells (R <= i < n-R on every axis) are written.  The
- *     R-thick boundary shell of both buffers is read-only for the whole run.
- *   - The caller owns every buffer.  No entry point allocates device memory
- *     on the sweep path; stencil_residual_norm and the mixed/batched entry
- *     points use a small workspace owned by the library (allocated once per
- *     device on first use).
- *   - Calls are stream-ordered.  They are re-entrant for disjoint buffers
- *     and not thread-safe on the same buffers.
+ *   - Only interior cells (R <= i < n-R on every axis) change.  The
+ *     R-thick boundary shell of both buffers keeps its bits for the whole
+ *     run; when both buffers hold the same shell (compared on the GPU at
+ *     the start of a dispatch) some kernels store shell cells of interior
+ *     rows with the value they already hold, so rows are written as whole
+ *     cache lines.
+ *   - The caller owns every buffer.  Entry points use a small device
+ *     workspace owned by the library (shell-comparison flags, reduction
+ *     partials, the batch table), allocated on first use per device and
+ *     stream and kept until the process ends.
+ *   - Calls are stream-ordered.  Calls on different streams never share
+ *     workspace; they are re-entrant for disjoint buffers and not
+ *     thread-safe on the same buffers.
  */
 #ifndef THESIS_AMD_STENCIL_ABI_H
 #define THESIS_AMD_STENCIL_ABI_H
@@ -226,8 +231,9 @@ typedef struct st_tile_rule {
     long long max_cells;
     int coarsen_y;        /* rows per tile frame (0 = as many as fit): the
                              plane-chunk form, used when every problem has
-                             radius 1..2 and (1+2R)*nx <= 4096; otherwise
-                             rows per wave of the march form (1, 2 or 4) */
+                             radius 1..2, (1+2R)*nx <= 4096, under 2^29
+                             cells and all share one op; otherwise rows per
+                             wave of the march form (1, 2 or 4) */
     int zchunk;           /* planes per tile (0 = whole z extent)         */
 } st_tile_rule;
 
@@ -239,7 +245,10 @@ typedef enum st_batch_mode {
 
 /* Advance every problem of the batch by its iteration count.  `stream`
  * must not be the null stream for ST_BATCH_GRAPH (a private stream is used
- * when it is NULL).  elapsed_ms (optional) covers the sweep launches only. */
+ * when it is NULL).  elapsed_ms (optional) covers the launches on the
+ * device: in the plane-chunk form the comparison of each problem's two
+ * shells (equal shells let whole rows be stored, shell cells with the value
+ * they hold) and the sweeps. */
 st_status stencil_dispatch_batched(const st_problem *problems, int nproblems,
                                    const st_tile_rule *rules, int nrules, int mode,
                                    void *stream, float *elapsed_ms);

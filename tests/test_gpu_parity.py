@@ -497,3 +497,36 @@ def test_round16_widen16_match_oracle(gpu, oracle, dtype):
     got_r = d_r.numpy()
     bad = np.flatnonzero(got_r != want_r)
     assert bad.size == 0, f"{bad.size} of {x.size} differ, e.g. {x[bad[:4]]} -> {got_r[bad[:4]]} vs {want_r[bad[:4]]}"
+
+
+@pytest.mark.parametrize("variant,form", [("coop", 3), ("smem_register", 2)])
+def test_two_streams_keep_their_own_shell_flags(gpu, oracle, variant, form):
+    """Dispatches issued back to back on two streams, one with equal shells
+    (fused / whole-row path) and one with different shells, each bit-exact:
+    the shell-comparison flags live in per-stream workspace."""
+    st = gpu
+    nx, ny, nz = 128, 40, 36
+    ua = oracle.init_seeded(nx, ny, nz, 3, 1, 0x5EED0031)
+    ua_rng = np.random.default_rng(5)
+    ua = np.where(ua == 0, ua_rng.random(ua.shape), ua).astype(np.float32)
+    ub = oracle.init_reference(nx, ny, nz, 3)
+    want_a = oracle.run(ua.copy(), ua.copy(), 3, 1, 1, 6).copy()
+    want_b = oracle.run(ub.copy(), np.zeros_like(ub), 3, 1, 1, 6).copy()
+    sa, sb = st.Stream(), st.Stream()
+    p = st.StencilParams.make(nx=nx, ny=ny, nz=nz, dimensions=3, radius=1, iterations=6, op=1,
+                              variant=variant, temporal_form=form, heuristic=1)
+    _announce("two streams", variant, form)
+    bufs = [st.DeviceArray.from_numpy(a) for a in (ua, ua, ub, np.zeros_like(ub))]
+    st.synchronize()
+    for _ in range(3):
+        st.dispatch_kernels(p, bufs[0], bufs[1], stream=sa)
+        st.dispatch_kernels(p, bufs[2], bufs[3], stream=sb)
+        sa.sync()
+        sb.sync()
+        assert_bitwise(bufs[0].numpy(), want_a, f"{variant} stream A (equal shells)")
+        assert_bitwise(bufs[2].numpy(), want_b, f"{variant} stream B (different shells)")
+        for d, a in zip(bufs, (ua, ua, ub, np.zeros_like(ub))):
+            d.upload(a)
+        st.synchronize()
+    for d in bufs:
+        d.free()

@@ -481,7 +481,7 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
     std::memcpy(host.data(), table.data(), table.size() * sizeof(BatchProblem));
     for (size_t i = 0; i < table.size(); ++i)
         std::memcpy(host.data() + tbytes + i * sizeof(int), &table[i].item0, sizeof(int));
-    if ((st = workspace_slot(1, host.size() + fbytes, &ws)) != ST_OK) return st;
+    if ((st = workspace_slot(1, host.size() + fbytes, s, &ws)) != ST_OK) return st;
     if ((st = hip_check(hipMemcpyAsync(ws, host.data(), host.size(), hipMemcpyHostToDevice, s),
                         "batch table copy")) != ST_OK)
         return st;

@@ -12,7 +12,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 
 #include "st_internal.hpp"
@@ -43,33 +45,40 @@ st_status hip_check(hipError_t e, const char *what)
 
 // ---------------------------------------------------------------- workspace
 
+// One workspace per (device, stream, slot): calls on different streams never
+// share device scratch (shell flags, reduction partials, batch tables), and
+// calls on one stream are ordered by the stream.
 static std::mutex g_ws_mutex;
-constexpr int kSlots = 4, kDevs = 64;
-static void *g_ws[kDevs][kSlots];
-static size_t g_ws_size[kDevs][kSlots];
+constexpr int kSlots = 4;
+struct WsBlock {
+    void *ptr = nullptr;
+    size_t size = 0;
+};
+static std::map<std::tuple<int, hipStream_t, int>, WsBlock> g_ws;
 
-st_status workspace_slot(int slot, size_t bytes, void **ptr)
+st_status workspace_slot(int slot, size_t bytes, hipStream_t stream, void **ptr)
 {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return hip_check(e, "hipGetDevice");
-    if (dev < 0 || dev >= kDevs) return fail(ST_EDEVICE, "device index %d out of range", dev);
     if (slot < 0 || slot >= kSlots) return fail(ST_EINVAL, "workspace slot %d", slot);
     std::lock_guard<std::mutex> lock(g_ws_mutex);
-    if (g_ws_size[dev][slot] < bytes) {
-        if (g_ws[dev][slot]) {
-            e = hipDeviceSynchronize();
+    WsBlock &w = g_ws[std::make_tuple(dev, stream, slot)];
+    if (w.size < bytes) {
+        if (w.ptr) {
+            // earlier work on this stream may still use the old block
+            e = hipStreamSynchronize(stream);
             if (e != hipSuccess) return hip_check(e, "workspace sync");
-            (void)hipFree(g_ws[dev][slot]);
-            g_ws[dev][slot] = nullptr;
-            g_ws_size[dev][slot] = 0;
+            (void)hipFree(w.ptr);
+            w.ptr = nullptr;
+            w.size = 0;
         }
         size_t sz = bytes < (1u << 20) ? (1u << 20) : bytes;
-        e = hipMalloc(&g_ws[dev][slot], sz);
+        e = hipMalloc(&w.ptr, sz);
         if (e != hipSuccess) return hip_check(e, "workspace hipMalloc");
-        g_ws_size[dev][slot] = sz;
+        w.size = sz;
     }
-    *ptr = g_ws[dev][slot];
+    *ptr = w.ptr;
     return ST_OK;
 }
 

@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void k_resid_final(const double *partial, int 
 st_status residual_norm(const Cfg &c, const void *u, double *norm, hipStream_t s)
 {
     void *ws = nullptr;
-    st_status st = workspace((kRedBlocks + 1) * sizeof(double), &ws);
+    st_status st = workspace((kRedBlocks + 1) * sizeof(double), s, &ws);
     if (st != ST_OK) return st;
     double *partial = static_cast<double *>(ws);
     Geom g;

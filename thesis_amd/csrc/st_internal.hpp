@@ -87,9 +87,10 @@ st_status residual_norm(const Cfg &c, const void *u, double *norm, hipStream_t s
 st_status run_mixed(const Cfg &c, void *x, void *e1, void *e2, void *d, void *u,
                     hipStream_t s);
 
-// Library-owned device workspace (grows on demand, one per device and slot).
-st_status workspace_slot(int slot, size_t bytes, void **ptr);
-inline st_status workspace(size_t bytes, void **ptr) { return workspace_slot(0, bytes, ptr); }
+// Library-owned device workspace (grows on demand, one per device, stream
+// and slot).
+st_status workspace_slot(int slot, size_t bytes, hipStream_t stream, void **ptr);
+inline st_status workspace(size_t bytes, hipStream_t stream, void **ptr) { return workspace_slot(0, bytes, stream, ptr); }
 
 st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules, int nrules,
                       int mode, hipStream_t s, float *elapsed_ms);

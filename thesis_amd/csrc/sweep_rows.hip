@@ -636,7 +636,7 @@ st_status rows_shell_check(const Cfg &c, const void *u1, const void *u2, hipStre
         return ST_OK;
     }
     void *ws = nullptr;
-    st_status st = workspace_slot(2, sizeof(int), &ws);
+    st_status st = workspace_slot(2, sizeof(int), s, &ws);
     if (st != ST_OK) return st;
     int *f = static_cast<int *>(ws);
     const int blocks = (int)std::max<long long>(1, std::min<long long>(2048, (n + 2047) / 2048));

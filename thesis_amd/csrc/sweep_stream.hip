@@ -513,7 +513,7 @@ st_status run_stream(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2
     *in_u2 = c.iters & 1;
     if (c.iters <= 0) return ST_OK;
     void *ws = nullptr;
-    st_status st = workspace_slot(3, kShellBlocks * sizeof(int), &ws);
+    st_status st = workspace_slot(3, kShellBlocks * sizeof(int), s, &ws);
     if (st != ST_OK) return st;
     hipLaunchKernelGGL(k_shell_eq, dim3(kShellBlocks), dim3(256), 0, s, static_cast<const float *>(u1),
                        static_cast<const float *>(u2), c.nx, c.ny, c.nz, static_cast<int *>(ws));

@@ -508,6 +508,7 @@ def main(argv=None):
     ap.add_argument("--weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                       "tuned", "unet64.npz"))
     ap.add_argument("--out", default=None)
+    ap.add_argument("--init", default=None, help="train: start from these weights (fine-tuning)")
     ap.add_argument("--fine", type=int, default=128)
     ap.add_argument("--problems", type=int, default=3)
     ap.add_argument("--tol", type=float, default=1e-6)
@@ -524,6 +525,8 @@ def main(argv=None):
         t_gen = time.perf_counter() - t0
         print(f"[warmstart] {a.count} solves of {a.n}^3 x {a.sweeps} sweeps: {t_gen:.1f} s", flush=True)
         net = UNet(a.batch, a.n, stream=s)
+        if a.init:
+            net.load(a.init)
         t0 = time.perf_counter()
         log = train(net, fields, a.count, a.steps, lr=a.lr, lr_final=a.lr / 10, log_every=100,
                     verbose=True, smooth_sweeps=a.smooth_sweeps, smooth_weight=a.smooth_weight)
