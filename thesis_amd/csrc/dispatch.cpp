@@ -171,15 +171,32 @@ static st_status resolve(const st_params *p, Cfg &c, bool mixed)
     c.gx = c.gy = c.gz = 1;
     if (mixed) return ST_OK;
     if (c.variant == ST_V_AUTO) {
-        // The streaming temporal form (two sweeps per HBM pass) where it
-        // applies and at least one fused pass runs; else the register march
-        // wherever it is instantiated; else the plain kernel.
+        // Measured on MI355X (FP32 R=1, warm, per sweep):
+        //   * the streaming temporal form (two sweeps per HBM pass) wins once
+        //     both fields exceed the 256 MiB last-level cache (384^3: 58 us
+        //     against 92 for the row kernel); below that the per-sweep row
+        //     kernel is served from that cache and wins (256^3: 22 against
+        //     32 us; 320^3 even);
+        //   * grids of up to about a million cells (3D) or four million (2D)
+        //     are bound by the launch cadence, where the plain kernel is the
+        //     cheapest (64^3: 2.5 us against 5.2 for the row kernel and 8.3
+        //     for the streaming form; 256^2: 2.5 against 7.5 for the 2D
+        //     register march);
+        //   * elsewhere the register march / row kernel wherever it is
+        //     instantiated, else the plain kernel.
         c.bx = c.by = c.bz = 0;
         c.heuristic = 1;
-        if (c.iters >= 4 && (c.tsteps == 0 || c.tsteps == 2) && c.form == 0 && stream_eligible(c)) {
+        const long long cells = (long long)c.nx * c.ny * c.nz;
+        const long long es = c.dtype == ST_F64 ? 8 : 4;
+        const long long llc_bytes = 256LL << 20;
+        const long long small = c.dims == 3 ? (1LL << 20) : (1LL << 22);
+        if (c.iters >= 4 && (c.tsteps == 0 || c.tsteps == 2) && c.form == 0 && stream_eligible(c) &&
+            2 * cells * es > llc_bytes) {
             c.variant = ST_V_COOP;
             c.cx = 1;
             c.cy = 0;
+        } else if (cells <= small) {
+            c.variant = ST_V_BASE;
         } else {
             const bool march_ok = (c.dtype == ST_F32 && c.r <= 8) ||
                                   (c.dtype == ST_F64 && (c.r == 1 || c.r == 2 || c.r == 4));
