@@ -530,3 +530,19 @@ def test_two_streams_keep_their_own_shell_flags(gpu, oracle, variant, form):
         st.synchronize()
     for d in bufs:
         d.free()
+
+
+@pytest.mark.parametrize("dims,n,its,want", [
+    (3, 64, 100, "base"),            # launch-bound: the plain kernel
+    (3, 128, 100, "smem_register"),  # fits the last-level cache: the row kernel
+    (3, 512, 100, "coop"),           # both fields exceed it: the streaming form
+    (3, 512, 2, "smem_register"),    # too few sweeps for a fused pass
+    (2, 256, 64, "base"),
+    (2, 4096, 64, "smem_register"),
+])
+def test_auto_choice_by_size(gpu, dims, n, its, want):
+    st = gpu
+    p = st.StencilParams.make(nx=n, ny=n, nz=n if dims == 3 else 1, dimensions=dims, radius=1,
+                              iterations=its, op=st.OP_JACOBI, variant="auto", heuristic=1)
+    got = {v: k for k, v in st.VARIANTS.items()}[st.launch_config(p)["variant"]]
+    assert got == want, (dims, n, its, got)

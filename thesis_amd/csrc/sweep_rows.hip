@@ -51,15 +51,18 @@ template <typename T> struct Quad;
 template <> struct Quad<float> { typedef float type __attribute__((ext_vector_type(4))); };
 template <> struct Quad<double> { typedef double type __attribute__((ext_vector_type(4))); };
 
+// Wave-wide rotates read a valid lane for every lane, so bound_ctrl (zero for
+// an invalid source lane) never applies; setting it lets the compiler drop
+// the copy of the old value.
 template <typename T>
 __device__ __forceinline__ T dpp_prev(T v)  // lane i <- lane i-1 (rotating)
 {
     if constexpr (sizeof(T) == 4) {
-        return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x13C, 0xF, 0xF, false));
+        return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x13C, 0xF, 0xF, true));
     } else {
         const long long b = __double_as_longlong(v);
-        const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x13C, 0xF, 0xF, false);
-        const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x13C, 0xF, 0xF, false);
+        const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x13C, 0xF, 0xF, true);
+        const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x13C, 0xF, 0xF, true);
         return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
     }
 }
@@ -68,11 +71,11 @@ template <typename T>
 __device__ __forceinline__ T dpp_next(T v)  // lane i <- lane i+1 (rotating)
 {
     if constexpr (sizeof(T) == 4) {
-        return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134, 0xF, 0xF, false));
+        return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134, 0xF, 0xF, true));
     } else {
         const long long b = __double_as_longlong(v);
-        const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x134, 0xF, 0xF, false);
-        const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x134, 0xF, 0xF, false);
+        const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x134, 0xF, 0xF, true);
+        const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x134, 0xF, 0xF, true);
         return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
     }
 }

@@ -86,14 +86,15 @@ __global__ __launch_bounds__(256) void k_shell_eq(const float *__restrict__ u1,
 }
 
 // Lane i receives v of lane i-1 / i+1, rotating at the wave ends (wave-wide
-// DPP rotates).
+// DPP rotates; every lane reads a valid lane, so bound_ctrl never applies and
+// only spares the copy of the old value).
 __device__ __forceinline__ float from_prev_lane(float v)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x13C, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x13C, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float from_next_lane(float v)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134, 0xF, 0xF, true));
 }
 
 // A lane's cells of one row: CX = 4*H values, quad h holding columns
