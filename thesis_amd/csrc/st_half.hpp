@@ -9,14 +9,11 @@
 
 namespace st {
 
-__device__ __forceinline__ uint16_t to_bf16(float f)
-{
-    uint32_t x = __float_as_uint(f);
-    if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40);
-    const uint32_t lsb = (x >> 16) & 1u;
-    x += 0x7fffu + lsb;
-    return (uint16_t)(x >> 16);
-}
+// BF16 rounding: the gfx950 conversion (v_cvt_pk_bf16_f32) rounds to nearest
+// even and keeps FP32 subnormals, which is the integer restatement
+// (x + 0x7fff + lsb) >> 16 for every non-NaN input, infinities and overflow
+// included; a NaN stays a NaN (the restatement sets its quiet bit).
+__device__ __forceinline__ uint16_t to_bf16(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
