@@ -132,7 +132,10 @@ typedef struct st_params {
                              3 = streaming rows, one wave per grid row (F32,
                              radius 1, nx a multiple of 4 up to 512,
                              cooperative launch; two fused sweeps
-                             when both buffers hold the same shell, else one).
+                             when both buffers hold the same shell, else one),
+                             4 = row frames of the SMEM_REGISTER row form, one
+                             sweep per grid barrier (3D F32/F64, radius 1..4,
+                             nx a multiple of 4).
                              SMEM_REGISTER: 0 = library's choice, 1 = register
                              march, 2 = row streaming (3D F32/F64, radius
                              1..4, nx a multiple of 4 up to 512; block_y =

@@ -113,3 +113,39 @@ def test_launch_config_shapes(st):
                                                   heuristic=1, op=st.OP_JACOBI))
     assert coop["temporal_steps"] >= 1 and coop["lds_bytes"] <= 160 * 1024
     assert coop["launches"] == 1
+
+
+def test_heuristic_keeps_coarsen_x(st):
+    """HEURISTIC picks the block shape; COARSEN_X still selects the unrolled
+    kernels (src/stencil_dispatch.cu:14-42, :148): the x grid shrinks by it."""
+    for variant in ("base", "smem", "smem_padded"):
+        for cx in (1, 2, 4, 8):
+            cfg = st.launch_config(st.StencilParams.make(dim=512, radius=1, iterations=4, variant=variant,
+                                                         heuristic=1, coarsen_x=cx))
+            assert cfg["coarsen_x"] == cx, (variant, cx)
+            assert cfg["grid_x"] == -(-512 // (cfg["block_x"] * cx)), (variant, cx)
+    # the padded tile halves its coarsening until it fits in LDS
+    big = st.launch_config(st.StencilParams.make(dim=256, radius=8, iterations=4, variant="smem_padded",
+                                                 heuristic=1, coarsen_x=8))
+    assert big["lds_bytes"] <= 160 * 1024 and big["coarsen_x"] < 8
+    # the library's choice for small grids runs one cell per thread
+    auto = st.launch_config(st.StencilParams.make(dim=64, radius=1, iterations=4, variant="auto"))
+    assert st.VARIANTS["base"] == auto["variant"] and auto["coarsen_x"] == 1
+
+
+def test_coop_row_form_config(st):
+    """COOP at radius > 1 runs the row kernel with a grid barrier after each
+    sweep: one launch for every sweep, or one per sweep with
+    launch_per_pass."""
+    for r in (2, 3, 4):
+        cfg = st.launch_config(st.StencilParams.make(dim=384, radius=r, iterations=10, variant="coop",
+                                                     heuristic=1))
+        assert cfg["launches"] == 1 and cfg["temporal_steps"] == 1 and cfg["block_x"] == 384
+        per = st.launch_config(st.StencilParams.make(dim=384, radius=r, iterations=10, variant="coop",
+                                                     heuristic=1, launch_per_pass=1))
+        assert per["launches"] == 10
+    f4 = st.launch_config(st.StencilParams.make(dim=128, radius=1, iterations=7, variant="coop",
+                                                temporal_form=4, heuristic=1))
+    assert f4["launches"] == 1 and f4["temporal_steps"] == 1
+    _raises(st, 5, variant="coop", temporal_form=4, nx=62, ny=40, nz=40, heuristic=1)
+    _raises(st, 1, variant="coop", temporal_form=5, dim=64, heuristic=1)

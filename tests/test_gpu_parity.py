@@ -404,6 +404,67 @@ def test_coop_stream_form_rejects_unsupported(gpu):
             st.check_early_exit(p)
 
 
+# COOP row form: the row kernel, every sweep in one cooperative launch with a
+# grid barrier after each (or one launch per sweep)
+COOP_ROWS_CASES = [
+    ((64, 40, 37), "f32", (1, 2, 3, 4)),
+    ((384, 30, 24), "f32", (2, 4)),        # 8 cells per lane
+    ((64, 33, 29), "f64", (1, 2, 3, 4)),
+    ((1000, 13, 9), "f32", (2,)),          # 4 waves per row
+    ((384, 21, 19), "f64", (4,)),          # two halo strips per wave
+]
+
+
+@pytest.mark.parametrize("shape,dt,radii", COOP_ROWS_CASES, ids=str)
+@pytest.mark.parametrize("op", [0, 1])
+@pytest.mark.parametrize("per_pass", [0, 1])
+def test_coop_rows_form(gpu, oracle, shape, dt, radii, op, per_pass):
+    st = gpu
+    nx, ny, nz = shape
+    npdt = np.float64 if dt == "f64" else np.float32
+    for r in radii:
+        u1 = oracle.init_seeded(nx, ny, nz, 3, r, 0x5EED0040 + r, dtype=npdt)
+        rng = np.random.default_rng(nx + 7 * r)
+        u1 = np.where(u1 == 0, rng.random(u1.shape), u1).astype(npdt)
+        # op 1 starts with equal shells (whole-row stores), op 0 with buffer 2 zero
+        u2 = np.zeros_like(u1) if op == 0 else u1.copy()
+        for its in (1, 2, 5):
+            _announce("coop rows form", shape, dt, r, op, per_pass, its)
+            a, b = u1.copy(), u2.copy()
+            want = oracle.run(a, b, 3, r, op, its)
+            other = b if want is a else a
+            p = st.StencilParams.make(nx=nx, ny=ny, nz=nz, dimensions=3, radius=r, iterations=its, op=op,
+                                      dtype=st.F64 if dt == "f64" else st.F32, variant="coop",
+                                      temporal_form=4, heuristic=1, launch_per_pass=per_pass)
+            cfg = st.launch_config(p)
+            assert cfg["launches"] == (its if per_pass else 1)
+            (got, got_other), in_u2 = run_gpu_both(st, p, u1, u2)
+            assert in_u2 == bool(its % 2)
+            assert_bitwise(got, want, f"coop rows {shape} {dt} r={r} op={op} its={its} per_pass={per_pass}")
+            assert_bitwise(got_other, other, f"coop rows other buffer {shape} {dt} r={r} op={op} its={its}")
+
+
+def test_coop_heuristic_radius2_is_row_form_and_matches(gpu, oracle):
+    """The library's COOP choice at radius 2 (row form, one launch) agrees
+    bit for bit with the per-sweep base kernel after 6 sweeps."""
+    st = gpu
+    nx, ny, nz = 192, 70, 66
+    u1 = oracle.init_reference(nx, ny, nz, 3)
+    u2 = np.zeros_like(u1)
+    _announce("coop heuristic r=2")
+    p = st.StencilParams.make(nx=nx, ny=ny, nz=nz, dimensions=3, radius=2, iterations=6, op=0,
+                              variant="coop", heuristic=1)
+    assert st.launch_config(p)["launches"] == 1
+    got, in_u2 = run_gpu(st, p, u1, u2)
+    q = st.StencilParams.make(nx=nx, ny=ny, nz=nz, dimensions=3, radius=2, iterations=6, op=0,
+                              variant="base", heuristic=1, coarsen_x=1)
+    ref, in_u2b = run_gpu(st, q, u1, u2)
+    assert in_u2 == in_u2b
+    assert_bitwise(got, ref, "coop r=2 against base")
+    want = oracle.run(u1.copy(), u2.copy(), 3, 2, 0, 6)
+    assert_bitwise(got, want, "coop r=2 against oracle")
+
+
 # SMEM_REGISTER row-streaming form (one sweep per launch, radius 1..4)
 ROWS_CASES = [
     ((64, 40, 37), "f32", (1, 2, 3, 4), {}),
@@ -499,7 +560,7 @@ def test_round16_widen16_match_oracle(gpu, oracle, dtype):
     assert bad.size == 0, f"{bad.size} of {x.size} differ, e.g. {x[bad[:4]]} -> {got_r[bad[:4]]} vs {want_r[bad[:4]]}"
 
 
-@pytest.mark.parametrize("variant,form", [("coop", 3), ("smem_register", 2)])
+@pytest.mark.parametrize("variant,form", [("coop", 3), ("coop", 4), ("smem_register", 2)])
 def test_two_streams_keep_their_own_shell_flags(gpu, oracle, variant, form):
     """Dispatches issued back to back on two streams, one with equal shells
     (fused / whole-row path) and one with different shells, each bit-exact:

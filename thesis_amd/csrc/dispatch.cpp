@@ -112,7 +112,7 @@ static st_status validate_common(const st_params *p, bool mixed)
     }
     if (p->coarsen_x < 0 || p->coarsen_y < 0 || p->smem_pad < 0 || p->temporal_steps < 0 ||
         p->launch_per_pass < 0 || p->launch_per_pass > 1 || p->temporal_form < 0 ||
-        p->temporal_form > 3)
+        p->temporal_form > 4)
         return fail(ST_EINVAL, "coarsening, padding, temporal steps, launch_per_pass or "
                                "temporal_form out of range");
     const bool heur = p->heuristic || p->block_x <= 0;
@@ -197,10 +197,12 @@ static st_status resolve(const st_params *p, Cfg &c, bool mixed)
             c.cy = 0;
         } else if (cells <= small) {
             c.variant = ST_V_BASE;
+            c.cx = 1;
         } else {
             const bool march_ok = (c.dtype == ST_F32 && c.r <= 8) ||
                                   (c.dtype == ST_F64 && (c.r == 1 || c.r == 2 || c.r == 4));
             c.variant = march_ok ? ST_V_SMEM_REGISTER : ST_V_BASE;
+            if (!march_ok) c.cx = 1;
         }
     }
     switch (c.variant) {
@@ -273,6 +275,7 @@ st_status stencil_launch_config(const st_params *p, st_launch_info *info)
             if (n && (n & 1) != (c.iters & 1)) ++n;
             if (c.kind == 2) n = c.iters / c.tsteps;
             if (c.kind == 3) n = c.iters > 0 ? 1 : 0;
+            if (c.kind == 5) n = c.iters;
             info->launches = n ? (c.per_pass ? n : 1) : 0;
         } else {
             info->launches = c.iters;

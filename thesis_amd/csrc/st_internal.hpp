@@ -64,6 +64,8 @@ st_status run_tblock(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2
 // SMEM_REGISTER row-streaming form, one sweep (sweep_rows.hip).
 bool rows_eligible(const Cfg &c);
 st_status resolve_rows(Cfg &c);
+st_status resolve_rows_coop(Cfg &c);
+st_status run_rows_coop(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2);
 st_status launch_rows(const Cfg &c, const void *in, void *out, hipStream_t s);
 // 16-bit storage modes on the geometry of `c` (FP32 arithmetic, JACOBI
 // weight c.weight): defect d = round16(w*sum(x) - x), else the correction

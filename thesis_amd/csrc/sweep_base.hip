@@ -194,16 +194,20 @@ static st_status finish_grid(Cfg &c)
     return ST_OK;
 }
 
+// HEURISTIC picks the workgroup shape only; coarsen_x is honoured as given
+// (src/stencil_dispatch.cu:14-42 selects the unrolled kernels from
+// COARSEN_X whatever the block shape), except that the padded tile halves it
+// until the tile fits in LDS.
 st_status resolve_base(Cfg &c)
 {
-    if (c.bx <= 0) { c.bx = 64; c.by = 4; c.bz = c.dims == 3 ? 1 : 1; c.cx = 1; }
+    if (c.bx <= 0) { c.bx = 64; c.by = 4; c.bz = 1; }
     c.lds = 0;
     return finish_grid(c);
 }
 
 st_status resolve_smem(Cfg &c)
 {
-    if (c.bx <= 0) { c.bx = 64; c.by = 4; c.bz = c.dims == 3 ? 4 : 1; c.cx = 1; }
+    if (c.bx <= 0) { c.bx = 64; c.by = 4; c.bz = c.dims == 3 ? 4 : 1; }
     const int es = c.dtype == ST_F64 ? 8 : 4;
     const long long lds = (long long)c.bx * c.cx * c.by * (c.dims == 3 ? c.bz : 1) * es;
     if (lds > kMaxLds) return fail(ST_ELDS, "SMEM tile needs %lld bytes of LDS", lds);
@@ -213,15 +217,19 @@ st_status resolve_smem(Cfg &c)
 
 st_status resolve_padded(Cfg &c)
 {
+    const int es = c.dtype == ST_F64 ? 8 : 4;
+    auto tile_lds = [&]() {
+        const long long px = (long long)c.bx * c.cx + 2 * c.r + c.pad;
+        return px * (c.by + 2 * c.r) * (c.dims == 3 ? c.bz + 2 * c.r : 1) * es;
+    };
     if (c.bx <= 0) {
-        c.bx = 64; c.by = 4; c.bz = c.dims == 3 ? 4 : 1; c.cx = 1;
+        c.bx = 64; c.by = 4; c.bz = c.dims == 3 ? 4 : 1;
         if (c.by < c.r) c.by = c.r;
         if (c.dims == 3 && c.bz < c.r) c.bz = c.r;
         while ((long long)c.bx * c.by * c.bz > kMaxThreads && c.bx > 1) c.bx /= 2;
+        while (tile_lds() > kMaxLds && c.cx > 1) c.cx /= 2;
     }
-    const int es = c.dtype == ST_F64 ? 8 : 4;
-    const long long px = (long long)c.bx * c.cx + 2 * c.r + c.pad;
-    const long long lds = px * (c.by + 2 * c.r) * (c.dims == 3 ? c.bz + 2 * c.r : 1) * es;
+    const long long lds = tile_lds();
     if (lds > kMaxLds) return fail(ST_ELDS, "SMEM_PADDED tile needs %lld bytes of LDS", lds);
     c.lds = (int)lds;
     return finish_grid(c);

@@ -23,6 +23,12 @@
 // buffers provide it (both buffers' shells are never written).  The number
 // of passes is chosen with the parity of the iteration count, so the result
 // lands in the same buffer as with per-sweep launches.
+//
+// resolve_coop picks among the COOP forms: streaming rows (two fused sweeps
+// per pass, sweep_stream.hip), register-queue frames (sweep_tblock.hip), the
+// row kernel with a grid barrier after every sweep (sweep_rows.hip, the
+// reference's own one-sweep-per-barrier schedule) and the LDS-ring tiles
+// above.
 #include <hip/hip_cooperative_groups.h>
 
 #include "st_device.hpp"
@@ -229,6 +235,11 @@ st_status resolve_coop(Cfg &c)
             return resolve_tblock(c, t);
         }
     }
+    // row form, one sweep per grid barrier: requested, or the library's
+    // choice for heuristic shapes where it applies (every other radius and
+    // FP64; measured on 512^3 FP32 R=2: 1.9 ms for 8 sweeps against 17 ms
+    // for the LDS-ring tiles)
+    if (c.form == 4 || (c.form == 0 && heur && c.tsteps <= 1 && rows_eligible(c))) return resolve_rows_coop(c);
     c.kind = 1;
     int want = c.tsteps > 0 ? c.tsteps : 2;
     if (want > c.iters && c.iters > 0) want = c.iters;
@@ -286,6 +297,7 @@ st_status run_coop(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2)
 {
     if (c.kind == 2) return run_tblock(c, u1, u2, s, in_u2);
     if (c.kind == 3) return run_stream(c, u1, u2, s, in_u2);
+    if (c.kind == 5) return run_rows_coop(c, u1, u2, s, in_u2);
     const void *fn = pick_coop(c);
     if (!fn) return fail(ST_EUNSUPPORTED, "no COOP kernel for radius %d", c.r);
     CoopArgs a;

@@ -26,10 +26,18 @@
 // registers before the current plane is processed.  Only interior cells are
 // stored; the output's shell is untouched.
 //
+// The same kernel serves the COOP variant (src/stencil_kernel_coop.cu:5-49:
+// every sweep in one launch, a grid-wide barrier after each): launched
+// cooperatively with `sweeps` > 1, each workgroup runs its tiles of a sweep,
+// waits at the grid barrier and starts the next sweep with the buffers
+// swapped.
+//
 // Two more modes serve the 16-bit storage scheme of mixed.hip (JACOBI, FP32
 // arithmetic, T = float): the correction sweep reads 16-bit e, widens it,
 // and writes round16(widen(d) + w*sum) (d read at the output cell), the
 // defect reads FP32 x and writes round16(w*sum - x).
+#include <hip/hip_cooperative_groups.h>
+
 #include "st_device.hpp"
 #include "st_half.hpp"
 #include "st_internal.hpp"
@@ -43,6 +51,7 @@ struct RowsArgs {
     int wy;         // core rows per frame
     int tiles_y, zchunk, ntiles;
     int nt;         // non-temporal stores
+    int sweeps;     // sweeps of this launch (> 1 only for cooperative launches)
     const int *eq;  // sweep mode: 1 when both buffers' shells are equal
                     // (edge quads then store whole, shell cells unchanged)
 };
@@ -106,7 +115,7 @@ constexpr int rows_max_threads()
 
 template <int OP, int R, int CX, typename T, int MODE = RM_SWEEP, bool F16 = false, int HS = 1>
 __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_rows(
-    const void *__restrict__ in_, void *__restrict__ out_, const uint16_t *__restrict__ aux, RowsArgs a,
+    const void *__restrict__ in0, void *__restrict__ out0, const uint16_t *__restrict__ aux, RowsArgs a,
     T kc)
 {
     static_assert(R >= 1 && R <= 4, "x neighbours come from the adjacent lane's quad");
@@ -150,7 +159,13 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
         for (int h = 0; h < H; ++h) hxo[i][h] = min(256 * (hp[i] * H + h) + 4 * lane, a.nx - 4);
     }
     const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int nsweeps = MODE == RM_SWEEP ? a.sweeps : 1;
 
+    for (int sw = 0; sw < nsweeps; ++sw) {
+    // odd sweeps read the first output and write the first input
+    const void *in_ = (sw & 1) ? static_cast<const void *>(out0) : in0;
+    void *out_ = (sw & 1) ? const_cast<void *>(in0) : out0;
+    if (sw > 0) cooperative_groups::this_grid().sync();
     for (int tile = (int)bid; tile < a.ntiles; tile += (int)gridDim.x) {
         const int tyi = tile % a.tiles_y, tzi = tile / a.tiles_y;
         const int fy0 = tyi * wy;
@@ -412,6 +427,7 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
         }
         __syncthreads();  // the next tile rewrites the LDS planes
     }
+    }
 }
 
 // ---------------------------------------------------------------- host side
@@ -573,8 +589,11 @@ static st_status rows_resolve_mode(Cfg &c, int mode, bool f16)
 
 st_status resolve_rows(Cfg &c) { return rows_resolve_mode(c, RM_SWEEP, false); }
 
+// sweeps > 1: one cooperative launch runs them all, the buffers swapped
+// after each (the grid is capped at the co-resident workgroups; the tile loop
+// strides over the rest)
 static st_status rows_launch(const Cfg &c, const void *fn, const void *in, void *out, const uint16_t *aux,
-                             hipStream_t s)
+                             hipStream_t s, int sweeps = 1)
 {
     RowsArgs a;
     a.nx = c.nx; a.ny = c.ny; a.nz = c.nz;
@@ -585,20 +604,33 @@ static st_status rows_launch(const Cfg &c, const void *fn, const void *in, void 
     a.zchunk = c.zchunk;
     a.ntiles = a.tiles_y * ((c.nz + c.zchunk - 1) / c.zchunk);
     a.nt = 1;
+    a.sweeps = sweeps;
     a.eq = c.shell_eq;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds);
     if (e != hipSuccess) return hip_check(e, "hipFuncSetAttribute(rows LDS)");
     const int nthreads = 64 * c.wx * c.wy;
-    if (c.dtype == ST_F64) {
-        double kd = c.op == OP_THESIS ? (double)(2 * c.dims * c.r) : c.weight;
-        void *args[] = {&in, &out, &aux, &a, &kd};
-        e = hipLaunchKernel(fn, dim3(a.ntiles), dim3(nthreads), args, c.lds, s);
-    } else {
-        float kf = c.op == OP_THESIS ? (float)(2 * c.dims * c.r) : (float)c.weight;
-        void *args[] = {&in, &out, &aux, &a, &kf};
-        e = hipLaunchKernel(fn, dim3(a.ntiles), dim3(nthreads), args, c.lds, s);
+    int grid = a.ntiles;
+    if (sweeps > 1) {
+        int per_cu = 0, dev = 0, cus = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, nthreads, c.lds);
+        if (e != hipSuccess) return hip_check(e, "occupancy query (rows)");
+        if ((e = hipGetDevice(&dev)) != hipSuccess) return hip_check(e, "hipGetDevice");
+        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return hip_check(e, "CU count");
+        if (per_cu < 1 || cus < 1) return fail(ST_ELDS, "row kernel does not fit on a CU (lds %d)", c.lds);
+        if ((long long)per_cu * cus < grid) grid = per_cu * cus;
     }
-    if (e != hipSuccess) return hip_check(e, "rows launch");
+    double kd = c.op == OP_THESIS ? (double)(2 * c.dims * c.r) : c.weight;
+    float kf = (float)kd;
+    void *karg = c.dtype == ST_F64 ? static_cast<void *>(&kd) : static_cast<void *>(&kf);
+    void *args[] = {&in, &out, &aux, &a, karg};
+    if (sweeps > 1) {
+        e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(nthreads), args, c.lds, s);
+        if (e != hipSuccess) return hip_check(e, "hipLaunchCooperativeKernel(rows)");
+    } else {
+        e = hipLaunchKernel(fn, dim3(grid), dim3(nthreads), args, c.lds, s);
+        if (e != hipSuccess) return hip_check(e, "rows launch");
+    }
     return ST_OK;
 }
 
@@ -608,6 +640,41 @@ st_status launch_rows(const Cfg &c, const void *in, void *out, hipStream_t s)
     const void *fn = rows_fn_mode(c, c.cx, RM_SWEEP, false, &threads, c.hs);
     if (!fn || 64 * c.wx * c.wy > threads) return fail(ST_EUNSUPPORTED, "row-streaming kernel missing");
     return rows_launch(c, fn, in, out, nullptr, s);
+}
+
+// COOP variant, row form: the row kernel's tiles, every sweep in one
+// cooperative launch with a grid barrier between sweeps (or one launch per
+// sweep with launch_per_pass).
+st_status resolve_rows_coop(Cfg &c)
+{
+    if (!rows_eligible(c))
+        return fail(ST_EUNSUPPORTED, "COOP row form needs a 3D F32/F64 grid, radius 1..4 and nx a multiple "
+                                     "of 4 whose rows fit a workgroup with its halo strips");
+    st_status s = rows_resolve_mode(c, RM_SWEEP, false);
+    if (s != ST_OK) return s;
+    c.kind = 5;
+    c.tsteps = 1;
+    return ST_OK;
+}
+
+st_status run_rows_coop(const Cfg &c0, void *u1, void *u2, hipStream_t s, int *in_u2)
+{
+    *in_u2 = c0.iters & 1;
+    if (c0.iters <= 0) return ST_OK;
+    Cfg c = c0;
+    int threads = 0;
+    const void *fn = rows_fn_mode(c, c.cx, RM_SWEEP, false, &threads, c.hs);
+    if (!fn || 64 * c.wx * c.wy > threads) return fail(ST_EUNSUPPORTED, "row-streaming kernel missing");
+    st_status st = ST_OK;
+    if (c.iters > 1) st = rows_shell_check(c, u1, u2, s, &c.shell_eq);
+    if (st != ST_OK) return st;
+    if (!c.per_pass) return rows_launch(c, fn, u1, u2, nullptr, s, c.iters);
+    void *a = u1, *b = u2;
+    for (int i = 0; i < c.iters && st == ST_OK; ++i) {
+        st = rows_launch(c, fn, a, b, nullptr, s);
+        void *t = a; a = b; b = t;
+    }
+    return st;
 }
 
 // Shell comparison of one grid: k_flag_one sets the flag, any block that
