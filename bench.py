@@ -188,9 +188,9 @@ def probe(st, np):
     st.copy_f32(d2, d1, N ** 3)
     st.synchronize()
     print(f"[probe] copy {N}^3 fp32 ({4 * N ** 3} bytes read, {4 * N ** 3} written)", flush=True)
-    for v, its, form in (("smem_register", 2, 0), ("base", 2, 0), ("smem_padded", 2, 0),
-                         ("coop", 2, 1), ("coop", SWEEPS, 3)):
-        q = p.copy(variant=v, iterations=its, temporal_form=form)
+    for v, its, form, per_pass in (("smem_register", 2, 0, 0), ("base", 2, 0, 0), ("smem_padded", 2, 0, 0),
+                                   ("coop", 2, 1, 0), ("coop", SWEEPS, 3, 1)):
+        q = p.copy(variant=v, iterations=its, temporal_form=form, launch_per_pass=per_pass)
         st.initialize_region(q, d1, d2, st.INIT_SEEDED, seed=SEED)
         st.dispatch_kernels(q, d1, d2)
         st.synchronize()
@@ -218,7 +218,8 @@ def main():
     ap.add_argument("--probe", action="store_true",
                     help="short run for counter collection: one streaming copy of a 512^3 "
                          "field, 2 sweeps of each single-sweep family and of the LDS-ring "
-                         "COOP form, then one 100-sweep launch of the streaming COOP form")
+                         "COOP form, then 100 sweeps of the streaming COOP form with one "
+                         "launch per pass (the headline's configuration)")
     ap.add_argument("--dim", type=int, default=N, help="grid edge (default: the 512^3 headline)")
     ap.add_argument("--radius", type=int, default=RADIUS)
     ap.add_argument("--sweeps", type=int, default=SWEEPS, help="sweeps per step (even)")
@@ -330,8 +331,10 @@ def main():
                      "traffic": load_traffic(variant_name) if measured_traffic else None,
                      "bytes_per_launch_algorithmic": bytes_launch,
                      "avg_launch_ms": round(avg_launch_s * 1e3, 5),
-                     "note": "temporal-blocked launches advance several sweeps per HBM pass, "
-                             "so their algorithmic fraction can exceed 1" if variant_name == "coop"
+                     "note": ("temporal-blocked passes advance two sweeps per HBM pass, so the "
+                              "algorithmic fraction can exceed 1; per-launch figures average over "
+                              "all launches of a step (with one launch per pass, those past the "
+                              "pass count return at once)") if variant_name == "coop"
                              else "one sweep per launch"},
     }
     d1.free()

@@ -130,9 +130,11 @@ typedef struct st_params {
                              2 = register-queue full-row frames (F32, radius
                              1, temporal_steps 2, even iteration count),
                              3 = streaming rows, one wave per grid row (F32,
-                             radius 1, nx a multiple of 4 up to 512,
-                             cooperative launch; two fused sweeps
-                             when both buffers hold the same shell, else one),
+                             radius 1, nx a multiple of 4 up to 512; two
+                             fused sweeps per pass when both buffers hold the
+                             same shell, else one; with launch_per_pass,
+                             `iterations` launches of which those past the
+                             pass count return at once),
                              4 = row frames of the SMEM_REGISTER row form, one
                              sweep per grid barrier (3D F32/F64, radius 1..4,
                              nx a multiple of 4).

@@ -339,7 +339,8 @@ STREAM_SHAPES = [
 @pytest.mark.parametrize("shape,knobs", STREAM_SHAPES, ids=str)
 @pytest.mark.parametrize("op", [0, 1])
 @pytest.mark.parametrize("init", ["seeded", "reference"])
-def test_coop_stream_form(gpu, oracle, shape, knobs, op, init):
+@pytest.mark.parametrize("per_pass", [0, 1])
+def test_coop_stream_form(gpu, oracle, shape, knobs, op, init, per_pass):
     st = gpu
     nx, ny, nz = shape
     if init == "reference":
@@ -356,12 +357,13 @@ def test_coop_stream_form(gpu, oracle, shape, knobs, op, init):
         kw.setdefault("block_y", 10)  # waves per frame
         kw["block_x"] = nx
     for its in (1, 2, 3, 4, 5, 6, 7, 12):
-        _announce("coop stream form", shape, knobs, op, init, its)
+        _announce("coop stream form", shape, knobs, op, init, per_pass, its)
         want = oracle.run(u1.copy(), u2.copy(), 3, 1, op, its).copy()
         p = st.StencilParams.make(nx=nx, ny=ny, nz=nz, dimensions=3, radius=1, iterations=its,
-                                  op=op, variant="coop", temporal_form=3, **kw)
+                                  op=op, variant="coop", temporal_form=3, launch_per_pass=per_pass, **kw)
         cfg = st.launch_config(p)
-        assert cfg["launches"] == 1
+        # one cooperative launch, or one launch per possible pass
+        assert cfg["launches"] == (its if per_pass else 1)
         (got, other), in_u2 = run_gpu_both(st, p, u1, u2)
         assert in_u2 == bool(its % 2)
         assert_bitwise(got, want, f"stream form {shape} {knobs} op={op} {init} its={its}")
@@ -397,7 +399,7 @@ def test_coop_stream_one_shell_cell_differs(gpu, oracle, face):
 def test_coop_stream_form_rejects_unsupported(gpu):
     st = gpu
     for kw in (dict(nx=62, ny=40, nz=40), dict(nx=520, ny=40, nz=40), dict(dim=40, radius=2),
-               dict(dim=40, dtype=st.F64), dict(dim=40, launch_per_pass=1)):
+               dict(dim=40, dtype=st.F64)):
         p = st.StencilParams.make(variant="coop", temporal_form=3, heuristic=1, op=1,
                                   iterations=4, **kw)
         with pytest.raises(st.StencilError):

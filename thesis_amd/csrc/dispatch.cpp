@@ -177,6 +177,10 @@ static st_status resolve(const st_params *p, Cfg &c, bool mixed)
         //     against 92 for the row kernel); below that the per-sweep row
         //     kernel is served from that cache and wins (256^3: 22 against
         //     32 us; 320^3 even);
+        //   * the streaming form runs one launch per pass: a grid-wide
+        //     barrier of a cooperative launch costs 25-30 us on this device,
+        //     a kernel boundary a few (512^3, 100 sweeps: 10.1 ms against
+        //     11.4 for one cooperative launch);
         //   * grids of up to about a million cells (3D) or four million (2D)
         //     are bound by the launch cadence, where the plain kernel is the
         //     cheapest (64^3: 2.5 us against 5.2 for the row kernel and 8.3
@@ -195,6 +199,7 @@ static st_status resolve(const st_params *p, Cfg &c, bool mixed)
             c.variant = ST_V_COOP;
             c.cx = 1;
             c.cy = 0;
+            c.per_pass = 1;
         } else if (cells <= small) {
             c.variant = ST_V_BASE;
             c.cx = 1;
@@ -274,7 +279,7 @@ st_status stencil_launch_config(const st_params *p, st_launch_info *info)
             int n = c.iters > 0 ? (c.iters + c.tsteps - 1) / c.tsteps : 0;
             if (n && (n & 1) != (c.iters & 1)) ++n;
             if (c.kind == 2) n = c.iters / c.tsteps;
-            if (c.kind == 3) n = c.iters > 0 ? 1 : 0;
+            if (c.kind == 3) n = c.iters > 0 ? (c.per_pass ? c.iters : 1) : 0;
             if (c.kind == 5) n = c.iters;
             info->launches = n ? (c.per_pass ? n : 1) : 0;
         } else {

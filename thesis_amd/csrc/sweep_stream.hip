@@ -46,6 +46,7 @@ struct SmArgs {
     int tiles_y, zchunk, ntiles;
     int iters;
     int n2, n1;           // equal shells: passes of two sweeps, then of one
+    int p0, p1;           // passes of this launch: [p0, min(p1, passes))
     const int *shell_eq;  // kShellBlocks flags from k_shell_eq
 };
 
@@ -381,8 +382,8 @@ __global__ __launch_bounds__((sm_max_threads<CX, RPW>())) void k_stream(float *u
     const bool same = __syncthreads_and(ok) != 0;
     const unsigned nblk = gridDim.x;
     const unsigned bid = xcd_remap(blockIdx.x, nblk);
-    const int npass = same ? a.n2 + a.n1 : a.iters;
-    for (int p = 0; p < npass; ++p) {
+    const int npass = min(same ? a.n2 + a.n1 : a.iters, a.p1);
+    for (int p = a.p0; p < npass; ++p) {
         const float *X = (p & 1) ? u2 : u1;
         float *Y = (p & 1) ? u1 : u2;
         if (same && p < a.n2) sm_pass<OP, CX, RPW, 2>(X, Y, a, kc, lds, bid, nblk, same);
@@ -401,7 +402,7 @@ static int stream_cx(const Cfg &c)
 
 bool stream_eligible(const Cfg &c)
 {
-    return c.dtype == ST_F32 && c.dims == 3 && c.r == 1 && c.iters >= 1 && !c.per_pass &&
+    return c.dtype == ST_F32 && c.dims == 3 && c.r == 1 && c.iters >= 1 &&
            c.nx >= 8 && c.ny >= 3 && c.nz >= 3 && stream_cx(c) != 0;
 }
 
@@ -460,7 +461,7 @@ st_status resolve_stream(Cfg &c)
 {
     if (!stream_eligible(c))
         return fail(ST_EUNSUPPORTED, "streaming temporal form needs FP32, 3D, radius 1, nx a multiple "
-                                     "of 4 up to 512, one cooperative launch");
+                                     "of 4 up to 512");
     const int cx = stream_cx(c);
     // rows per wave: 2, or coarsen_y = 4 with 4 cells per lane (nx <= 256)
     int rpw = 2;
@@ -547,8 +548,22 @@ st_status run_stream(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2
     if (grid > a.ntiles) grid = a.ntiles;
     float kf = c.op == OP_THESIS ? (float)(2 * c.dims * c.r) : (float)c.weight;
     void *args[] = {&u1, &u2, &a, &kf};
-    e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, c.lds, s);
-    if (e != hipSuccess) return hip_check(e, "hipLaunchCooperativeKernel(stream)");
+    if (!c.per_pass) {
+        a.p0 = 0;
+        a.p1 = c.iters;
+        e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, c.lds, s);
+        if (e != hipSuccess) return hip_check(e, "hipLaunchCooperativeKernel(stream)");
+        return ST_OK;
+    }
+    // one launch per pass, kernel boundaries as the barrier; the pass count
+    // (iters, or n2 + n1 with equal shells) is known on the device only, so
+    // launches past it return at once
+    for (int p = 0; p < c.iters; ++p) {
+        a.p0 = p;
+        a.p1 = p + 1;
+        e = hipLaunchKernel(fn, dim3(grid), dim3(threads), args, c.lds, s);
+        if (e != hipSuccess) return hip_check(e, "stream pass launch");
+    }
     return ST_OK;
 }
 
