@@ -599,6 +599,8 @@ def test_two_streams_keep_their_own_shell_flags(gpu, oracle, variant, form):
     (3, 64, 100, "base"),            # launch-bound: the plain kernel
     (3, 128, 100, "smem_register"),  # fits the last-level cache: the row kernel
     (3, 512, 100, "coop"),           # both fields exceed it: the streaming form
+    (3, 320, 100, "coop"),           # both fields above 192 MiB
+    (3, 288, 100, "smem_register"),  # below: the row kernel
     (3, 512, 2, "smem_register"),    # too few sweeps for a fused pass
     (2, 256, 64, "base"),
     (2, 4096, 64, "smem_register"),

@@ -172,15 +172,18 @@ static st_status resolve(const st_params *p, Cfg &c, bool mixed)
     if (mixed) return ST_OK;
     if (c.variant == ST_V_AUTO) {
         // Measured on MI355X (FP32 R=1, warm, per sweep):
-        //   * the streaming temporal form (two sweeps per HBM pass) wins once
-        //     both fields exceed the 256 MiB last-level cache (384^3: 58 us
-        //     against 92 for the row kernel); below that the per-sweep row
-        //     kernel is served from that cache and wins (256^3: 22 against
-        //     32 us; 320^3 even);
-        //   * the streaming form runs one launch per pass: a grid-wide
-        //     barrier of a cooperative launch costs 25-30 us on this device,
-        //     a kernel boundary a few (512^3, 100 sweeps: 10.1 ms against
-        //     11.4 for one cooperative launch);
+        //   * the streaming temporal form runs one launch per pass: a
+        //     grid-wide barrier of a cooperative launch costs 25-30 us on
+        //     this device, a kernel boundary a few (512^3, 100 sweeps: 10.1 ms
+        //     against 11.4 for one cooperative launch);
+        //   * it wins over the per-sweep row kernel from 320^3 up, with equal
+        //     shells (two sweeps per pass; 320^3: 1065 against 771 GCells/s)
+        //     and with different shells (one sweep per pass; 622 against
+        //     567); at 288^3 and below the row kernel, served from the
+        //     256 MiB last-level cache, wins with different shells (288^3:
+        //     660 against 554) though not with equal ones (739 against 935),
+        //     and the shells are compared on the device only, so the switch
+        //     sits between 288^3 and 320^3 (both fields above 192 MiB);
         //   * grids of up to about a million cells (3D) or four million (2D)
         //     are bound by the launch cadence, where the plain kernel is the
         //     cheapest (64^3: 2.5 us against 5.2 for the row kernel and 8.3
@@ -192,10 +195,10 @@ static st_status resolve(const st_params *p, Cfg &c, bool mixed)
         c.heuristic = 1;
         const long long cells = (long long)c.nx * c.ny * c.nz;
         const long long es = c.dtype == ST_F64 ? 8 : 4;
-        const long long llc_bytes = 256LL << 20;
+        const long long stream_bytes = 192LL << 20;
         const long long small = c.dims == 3 ? (1LL << 20) : (1LL << 22);
         if (c.iters >= 4 && (c.tsteps == 0 || c.tsteps == 2) && c.form == 0 && stream_eligible(c) &&
-            2 * cells * es > llc_bytes) {
+            2 * cells * es > stream_bytes) {
             c.variant = ST_V_COOP;
             c.cx = 1;
             c.cy = 0;
