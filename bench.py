@@ -332,9 +332,10 @@ def main():
                      "bytes_per_launch_algorithmic": bytes_launch,
                      "avg_launch_ms": round(avg_launch_s * 1e3, 5),
                      "note": ("temporal-blocked passes advance two sweeps per HBM pass, so the "
-                              "algorithmic fraction can exceed 1; per-launch figures average over "
-                              "all launches of a step (with one launch per pass, those past the "
-                              "pass count return at once)") if variant_name == "coop"
+                              "algorithmic fraction can exceed 1; a launch is one pass, and the "
+                              "event time per launch includes the shell comparison and the band "
+                              "fix-up launches (which return at once with equal shells)")
+                             if variant_name == "coop"
                              else "one sweep per launch"},
     }
     d1.free()

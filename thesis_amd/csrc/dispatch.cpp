@@ -282,7 +282,9 @@ st_status stencil_launch_config(const st_params *p, st_launch_info *info)
             int n = c.iters > 0 ? (c.iters + c.tsteps - 1) / c.tsteps : 0;
             if (n && (n & 1) != (c.iters & 1)) ++n;
             if (c.kind == 2) n = c.iters / c.tsteps;
-            if (c.kind == 3) n = c.iters > 0 ? (c.per_pass ? c.iters : 1) : 0;
+            // streaming form per pass: n2 fused passes, then iters mod 4
+            // single ones (the band fix-up kernels are not counted)
+            if (c.kind == 3) n = c.iters > 0 ? (c.per_pass ? (c.iters - c.iters % 4) / 2 + c.iters % 4 : 1) : 0;
             if (c.kind == 5) n = c.iters;
             info->launches = n ? (c.per_pass ? n : 1) : 0;
         } else {

@@ -47,6 +47,8 @@ struct SmArgs {
     int iters;
     int n2, n1;           // equal shells: passes of two sweeps, then of one
     int p0, p1;           // passes of this launch: [p0, min(p1, passes))
+    int fuse_diff;        // fuse two sweeps with different shells too (the
+                          // host runs k_stream_fix after each such pass)
     const int *shell_eq;  // kShellBlocks flags from k_shell_eq
 };
 
@@ -382,17 +384,143 @@ __global__ __launch_bounds__((sm_max_threads<CX, RPW>())) void k_stream(float *u
     const bool same = __syncthreads_and(ok) != 0;
     const unsigned nblk = gridDim.x;
     const unsigned bid = xcd_remap(blockIdx.x, nblk);
-    const int npass = min(same ? a.n2 + a.n1 : a.iters, a.p1);
+    const bool fuse = same || a.fuse_diff;
+    const int npass = min(fuse ? a.n2 + a.n1 : a.iters, a.p1);
     for (int p = a.p0; p < npass; ++p) {
         const float *X = (p & 1) ? u2 : u1;
         float *Y = (p & 1) ? u1 : u2;
-        if (same && p < a.n2) sm_pass<OP, CX, RPW, 2>(X, Y, a, kc, lds, bid, nblk, same);
+        if (fuse && p < a.n2) sm_pass<OP, CX, RPW, 2>(X, Y, a, kc, lds, bid, nblk, same);
         else sm_pass<OP, CX, RPW, 1>(X, Y, a, kc, lds, bid, nblk, same);
         if (p + 1 < npass) cooperative_groups::this_grid().sync();
     }
 }
 
+// Fused passes with different shells.  Level 0 of a fused pass is the state
+// after an even number of sweeps, whose shell is u1's; level 1 has u2's
+// shell.  The fused pass takes every shell value from the buffer it reads,
+// so its level-2 result is exact except within two cells of the shell
+// (level 1 is wrong next to the shell when the pass reads u2, and level 1's
+// own shell is wrong when it reads u1).  k_stream_fix recomputes those
+// cells, the interior cells with an index 1, 2, n-3 or n-2 on some axis,
+// from level 0 exactly as the per-sweep ping-pong does: level-0 shell from
+// u1, interior from X; level-1 shell from u2, interior by the same sums in
+// the same order as every sweep kernel.  Both shells stay untouched (the
+// fused pass stores interior cells only when the shells differ).
+struct FixArgs {
+    int nx, ny, nz;
+    int kb[4], jb[4], ib[4];  // band indices per axis (1, 2, n-3, n-2, deduplicated)
+    int nkb, njb, nib;
+    long long plane_a;        // cells of a band plane: (nx-2)(ny-2)
+    long long plane_b;        // band cells of an inner plane
+    long long total;
+};
+
+template <int OP>
+__global__ __launch_bounds__(256) void k_stream_fix(const float *__restrict__ X, const float *__restrict__ S0,
+                                                    const float *__restrict__ S1, float *__restrict__ Y,
+                                                    FixArgs f, const int *__restrict__ shell_eq, float kc)
+{
+    int ok = 1;
+    for (int i = threadIdx.x; i < kShellBlocks; i += blockDim.x) ok &= shell_eq[i];
+    if (__syncthreads_and(ok)) return;  // equal shells: the fused pass was exact
+    const int nx = f.nx, ny = f.ny, nz = f.nz;
+    const long long sy = nx, sz = (long long)nx * ny, last = sz * nz - 1;
+    const int wx = nx - 2;
+    for (long long t = blockIdx.x * 256LL + threadIdx.x; t < f.total; t += 256LL * gridDim.x) {
+        int i, j, k;
+        if (t < f.nkb * f.plane_a) {
+            // planes k in the band: every interior cell
+            const int q = (int)(t / f.plane_a);
+            const long long r = t - q * f.plane_a;
+            k = f.kb[q];
+            j = 1 + (int)(r / wx);
+            i = 1 + (int)(r % wx);
+        } else {
+            // inner planes: band rows whole, inner rows at their band columns
+            const long long t2 = t - f.nkb * f.plane_a;
+            const long long q = t2 / f.plane_b;
+            long long r = t2 - q * f.plane_b;
+            k = 3 + (int)q;
+            if (r < (long long)f.njb * wx) {
+                j = f.jb[r / wx];
+                i = 1 + (int)(r % wx);
+            } else {
+                r -= (long long)f.njb * wx;
+                j = 3 + (int)(r / f.nib);
+                i = f.ib[r % f.nib];
+            }
+        }
+        // shell flags of the offsets -2..2 on each axis (the 25 level-0
+        // cells within distance 2 of the cell; offsets reaching outside the
+        // grid are never read: they belong to shell neighbours, whose level-1
+        // value is the stored shell)
+        bool xs[5], ys[5], zs[5];
+#pragma unroll
+        for (int d = 0; d < 5; ++d) {
+            xs[d] = i + d - 2 == 0 || i + d - 2 == nx - 1;
+            ys[d] = j + d - 2 == 0 || j + d - 2 == ny - 1;
+            zs[d] = k + d - 2 == 0 || k + d - 2 == nz - 1;
+        }
+        const long long base = i + (long long)j * sy + k * sz;
+        auto lev0 = [&](int dx, int dy, int dz) -> float {
+            const bool sh = xs[dx + 2] || ys[dy + 2] || zs[dz + 2];
+            // clamped into the field even where a compiler might read ahead
+            // of the shell test in lev1
+            const long long id = min(max(base + dx + dy * sy + dz * sz, 0LL), last);
+            return (sh ? S0 : X)[id];
+        };
+        auto lev1 = [&](int dx, int dy, int dz) -> float {
+            if (xs[dx + 2] || ys[dy + 2] || zs[dz + 2]) return S1[base + dx + dy * sy + dz * sz];
+            float u = 0.0f;
+            u += lev0(dx - 1, dy, dz);
+            u += lev0(dx + 1, dy, dz);
+            u += lev0(dx, dy - 1, dz);
+            u += lev0(dx, dy + 1, dz);
+            u += lev0(dx, dy, dz - 1);
+            u += lev0(dx, dy, dz + 1);
+            return finish<OP>(u, lev0(dx, dy, dz), kc);
+        };
+        float u = 0.0f;
+        u += lev1(-1, 0, 0);
+        u += lev1(1, 0, 0);
+        u += lev1(0, -1, 0);
+        u += lev1(0, 1, 0);
+        u += lev1(0, 0, -1);
+        u += lev1(0, 0, 1);
+        Y[base] = finish<OP>(u, lev1(0, 0, 0), kc);
+    }
+}
+
 // ---------------------------------------------------------------- host side
+
+// Band indices of one axis of length n (interior 1..n-2): 1, 2, n-3, n-2,
+// deduplicated; the inner indices 3..n-4 are the rest.
+static int band_axis(int n, int *b)
+{
+    int cnt = 0;
+    const int cand[4] = {1, 2, n - 3, n - 2};
+    for (int v : cand) {
+        if (v < 1 || v > n - 2) continue;
+        bool dup = false;
+        for (int q = 0; q < cnt; ++q) dup |= b[q] == v;
+        if (!dup) b[cnt++] = v;
+    }
+    return cnt;
+}
+
+static FixArgs fix_args(const Cfg &c)
+{
+    FixArgs f{};
+    f.nx = c.nx; f.ny = c.ny; f.nz = c.nz;
+    f.nkb = band_axis(c.nz, f.kb);
+    f.njb = band_axis(c.ny, f.jb);
+    f.nib = band_axis(c.nx, f.ib);
+    const long long inner_k = c.nz > 6 ? c.nz - 6 : 0, inner_j = c.ny > 6 ? c.ny - 6 : 0;
+    f.plane_a = (long long)(c.nx - 2) * (c.ny - 2);
+    f.plane_b = (long long)f.njb * (c.nx - 2) + inner_j * f.nib;
+    f.total = f.nkb * f.plane_a + inner_k * f.plane_b;
+    return f;
+}
 
 static int stream_cx(const Cfg &c)
 {
@@ -549,20 +677,39 @@ st_status run_stream(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2
     float kf = c.op == OP_THESIS ? (float)(2 * c.dims * c.r) : (float)c.weight;
     void *args[] = {&u1, &u2, &a, &kf};
     if (!c.per_pass) {
+        // one cooperative launch: different shells run one sweep per pass
         a.p0 = 0;
         a.p1 = c.iters;
+        a.fuse_diff = 0;
         e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, c.lds, s);
         if (e != hipSuccess) return hip_check(e, "hipLaunchCooperativeKernel(stream)");
         return ST_OK;
     }
-    // one launch per pass, kernel boundaries as the barrier; the pass count
-    // (iters, or n2 + n1 with equal shells) is known on the device only, so
-    // launches past it return at once
-    for (int p = 0; p < c.iters; ++p) {
+    // one launch per pass, kernel boundaries as the barrier; n2 fused passes
+    // whatever the shells, each followed by the band fix-up (which returns
+    // at once when the shells are equal), then n1 single-sweep passes
+    a.fuse_diff = 1;
+    const FixArgs f = fix_args(c);
+    const void *fix = c.op == OP_THESIS ? reinterpret_cast<const void *>(&k_stream_fix<OP_THESIS>)
+                                        : reinterpret_cast<const void *>(&k_stream_fix<OP_JACOBI>);
+    long long fix_blocks = (f.total + 255) / 256;
+    if (fix_blocks > 1024) fix_blocks = 1024;
+    if (fix_blocks < 1) fix_blocks = 1;
+    const float *s0 = static_cast<const float *>(u1), *s1 = static_cast<const float *>(u2);
+    const int *flags = a.shell_eq;
+    for (int p = 0; p < a.n2 + a.n1; ++p) {
         a.p0 = p;
         a.p1 = p + 1;
         e = hipLaunchKernel(fn, dim3(grid), dim3(threads), args, c.lds, s);
         if (e != hipSuccess) return hip_check(e, "stream pass launch");
+        if (p < a.n2) {
+            const float *x = static_cast<const float *>((p & 1) ? u2 : u1);
+            float *y = static_cast<float *>((p & 1) ? u1 : u2);
+            FixArgs fa = f;
+            void *fargs[] = {&x, &s0, &s1, &y, &fa, &flags, &kf};
+            e = hipLaunchKernel(fix, dim3((unsigned)fix_blocks), dim3(256), fargs, 0, s);
+            if (e != hipSuccess) return hip_check(e, "stream band fix-up launch");
+        }
     }
     return ST_OK;
 }
