@@ -131,10 +131,11 @@ typedef struct st_params {
                              1, temporal_steps 2, even iteration count),
                              3 = streaming rows, one wave per grid row (F32,
                              radius 1, nx a multiple of 4 up to 512; two
-                             fused sweeps per pass when both buffers hold the
-                             same shell, else one; with launch_per_pass,
-                             `iterations` launches of which those past the
-                             pass count return at once),
+                             fused sweeps per pass; as one cooperative launch
+                             only when both buffers hold the same shell (else
+                             one sweep per pass); with launch_per_pass one
+                             launch per pass whatever the shells, a band
+                             fix-up kernel after each fused pass),
                              4 = row frames of the SMEM_REGISTER row form, one
                              sweep per grid barrier (3D F32/F64, radius 1..4,
                              nx a multiple of 4).
