@@ -45,6 +45,35 @@ def _cpu_model() -> str:
     return "unknown"
 
 
+def warm_start_leg(problems: int = 5, coarse_sweeps: int = 2000):
+    """The metric's second half: wall time to relative residual 1e-6 on
+    128^3 Laplace problems with Fourier-mode faces, learned warm start
+    (the tuned U-Net's 64^3 prediction, upsampled) against the plain zero
+    start, and the same with 2000 coarse JACOBI sweeps before the upsampling
+    (and that refinement of the network's input guess alone, the control)."""
+    import numpy as np
+    from thesis_amd import warmstart
+
+    weights = os.path.join(ROOT, "thesis_amd", "tuned", "unet64.npz")
+    with np.load(weights, allow_pickle=False) as z:
+        state = {k: z[k] for k in z.files}
+    rows = []
+    for i in range(problems):
+        r = warmstart.compare_warm_start(state, seed=0x5EED0005 + 1_000_000 + i, n=128, m=64, tol=1e-6,
+                                         check_every=200, coarse_sweeps=coarse_sweeps)
+        rows.append({"plain_sweeps": r["plain"]["sweeps"], "plain_s": round(r["plain"]["seconds"], 4),
+                     "learned_sweeps": r["learned"]["sweeps"],
+                     "learned_ratio": round(r["time_ratio_learned_vs_plain"], 4),
+                     "learned_coarse_ratio": round(r["time_ratio_learned_coarse_vs_plain"], 4),
+                     "blend_coarse_ratio": round(r["time_ratio_blend_coarse_vs_plain"], 4)})
+    mean = lambda k: round(float(np.mean([r[k] for r in rows])), 4)
+    return {"metric": "time to relative residual 1e-6 at 128^3: learned warm start / plain zero start",
+            "unit": "ratio", "higher_is_better": False, "problems": problems,
+            "learned": mean("learned_ratio"), "learned_coarse": mean("learned_coarse_ratio"),
+            "blend_coarse": mean("blend_coarse_ratio"), "coarse_sweeps": coarse_sweeps,
+            "per_problem": rows}
+
+
 def cpu_baseline(budget_s: float = 12.0, single_budget_s: float = 6.0):
     """Time the CPU oracle on sweeps of the same field (FP32) until the
     budget is used (at least 2 sweeps): OpenMP over all host threads (the
@@ -208,6 +237,8 @@ def main():
     ap.add_argument("--block", default="", help="bx,by,bz override (heuristic otherwise)")
     ap.add_argument("--coarsen", default="", help="cx,cy override")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-warm", action="store_true",
+                    help="skip the learned warm-start leg of the headline line")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--workload", default="sweep512",
                     choices=["sweep512", "batched", "mixed384"],
@@ -340,6 +371,8 @@ def main():
     }
     d1.free()
     d2.free()
+    if headline and not args.no_warm:
+        out["warm_start"] = warm_start_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_budget)
     print(json.dumps(out), flush=True)
