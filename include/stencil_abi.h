@@ -157,7 +157,9 @@ typedef struct st_launch_info {
     int zchunk;                  /* planes per workgroup (SMEM_REGISTER)       */
     int temporal_steps;          /* sweeps per pass (COOP)                     */
     int lds_bytes;               /* LDS per workgroup                          */
-    int launches;                /* kernel launches for `iterations` sweeps    */
+    int launches;                /* sweep-kernel launches for `iterations`
+                                    sweeps (the shell comparison and the
+                                    streaming form's band fix-ups are extra) */
 } st_launch_info;
 
 /* Default knobs of include/constants.h:1-52: DIM 256, DIMENSIONS 3,
