@@ -621,6 +621,31 @@ st_status resolve_stream(Cfg &c)
                 const SmPlan q = stream_plan(c, wy, rpw, cus, chunk_req);
                 if (q.cost < best.cost * 0.97) best = q;  // fewer waves must pay for themselves
             }
+        // Fields of three times the 256 MiB last-level cache and more are
+        // bound by HBM: there the z chunks' halo planes (re-read by the next
+        // chunk) cost more than idle CUs, as long as 75-92% of the CUs hold
+        // a tile.  Measured at 512^3: 8 waves x 5 chunks of 103 planes (215
+        // tiles) 9.67 ms per 100 sweeps, 10 x 8 chunks of 64 (256 tiles, the
+        // busiest-CU model's choice) 10.2, 12 x 8 of 64 (208) 9.88.
+        if (chunk_req <= 0 && 2LL * c.nx * c.ny * c.nz * 4 >= (768LL << 20)) {
+            SmPlan pick{};
+            double pick_amp = 0;
+            for (int wy = wmax; wy >= (wmax * 2) / 3 && wy >= wmin; --wy) {
+                const int core_h = wy * rpw - 4;
+                const int tiles_y = (c.ny + core_h - 1) / core_h;
+                for (int nch = 1; nch * tiles_y <= cus; ++nch) {
+                    const int chunk = (c.nz + nch - 1) / nch;
+                    const int ntiles = tiles_y * ((c.nz + chunk - 1) / chunk);
+                    if (chunk < 4 || ntiles * 100 < cus * 75 || ntiles * 100 > cus * 92) continue;
+                    const double amp = (chunk + 4.0) / chunk;
+                    if (pick.wy == 0 || amp < pick_amp - 1e-9) {
+                        pick = stream_plan(c, wy, rpw, cus, chunk);
+                        pick_amp = amp;
+                    }
+                }
+            }
+            if (pick.wy) best = pick;
+        }
     }
     c.kind = 3;
     c.cx = cx;
