@@ -621,12 +621,13 @@ st_status resolve_stream(Cfg &c)
                 const SmPlan q = stream_plan(c, wy, rpw, cus, chunk_req);
                 if (q.cost < best.cost * 0.97) best = q;  // fewer waves must pay for themselves
             }
-        // Fields of three times the 256 MiB last-level cache and more are
-        // bound by HBM: there the z chunks' halo planes (re-read by the next
-        // chunk) cost more than idle CUs, as long as 75-92% of the CUs hold
-        // a tile.  Measured at 512^3: 8 waves x 5 chunks of 103 planes (215
-        // tiles) 9.67 ms per 100 sweeps, 10 x 8 chunks of 64 (256 tiles, the
-        // busiest-CU model's choice) 10.2, 12 x 8 of 64 (208) 9.88.
+        // Fields of three times the 256 MiB last-level cache and more: the
+        // plan with the fewest planes marched per plane stored ((chunk + 4)
+        // / chunk) wins as long as 75-92% of the CUs hold a tile (frames of
+        // 2/3 of the largest height or more, as measured).  At 512^3: 8
+        // waves x 5 chunks of 103 planes (215 tiles) 9.67 ms per 100 sweeps,
+        // 10 x 8 chunks of 64 (256 tiles, the busiest-CU model's choice)
+        // 10.2, 12 x 8 of 64 (208) 9.88; the HBM bytes are the same.
         if (chunk_req <= 0 && 2LL * c.nx * c.ny * c.nz * 4 >= (768LL << 20)) {
             SmPlan pick{};
             double pick_amp = 0;
