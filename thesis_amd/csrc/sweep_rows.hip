@@ -565,12 +565,25 @@ static st_status rows_resolve_mode(Cfg &c, int mode, bool f16)
     const int tiles_y = (c.ny + wy - 1) / wy;
     int chunk = (!c.heuristic && c.bz > 0) ? c.bz : 0;
     if (chunk <= 0) {
-        // one tile per CU (the LDS planes allow one workgroup per CU)
-        int chunks = cus / tiles_y;
-        if (chunks < 1) chunks = 1;
-        chunk = (c.nz + chunks - 1) / chunks;
-        if (chunk < 2 * c.r) chunk = 2 * c.r;
-        if (chunk > c.nz) chunk = c.nz;
+        // The z chunk that minimises the busiest CU's work: rounds of tiles
+        // (one workgroup per CU) x planes marched per tile (chunk + 2R).
+        // Measured at 384^3 R=4: FP32 (48 frames) 5 chunks of 77, one
+        // round, 2.54 ms per 20 sweeps; FP64 (96 frames) 8 chunks of 48,
+        // three full rounds, 5.38 ms, against 6.19 for 2 chunks of 192
+        // (one round with 64 CUs idle); the model orders all seven chunk
+        // sizes tried at both precisions as measured.
+        long long best_cost = -1;
+        for (int nch = 1; nch <= c.nz; ++nch) {
+            const int ch = (c.nz + nch - 1) / nch;
+            if (ch < 2 * c.r) break;
+            const long long nt = (long long)tiles_y * ((c.nz + ch - 1) / ch);
+            const long long cost = (nt + cus - 1) / cus * (ch + 2 * c.r);
+            if (best_cost < 0 || cost < best_cost) {
+                best_cost = cost;
+                chunk = ch;
+            }
+        }
+        if (chunk <= 0) chunk = c.nz;
     }
     c.kind = 4;
     c.cx = cx;
