@@ -621,14 +621,16 @@ st_status resolve_stream(Cfg &c)
                 const SmPlan q = stream_plan(c, wy, rpw, cus, chunk_req);
                 if (q.cost < best.cost * 0.97) best = q;  // fewer waves must pay for themselves
             }
-        // Fields of three times the 256 MiB last-level cache and more: the
+        // Fields of 2.5 times the 256 MiB last-level cache and more: the
         // plan with the fewest planes marched per plane stored ((chunk + 4)
         // / chunk) wins as long as 75-92% of the CUs hold a tile (frames of
-        // 2/3 of the largest height or more, as measured).  At 512^3: 8
-        // waves x 5 chunks of 103 planes (215 tiles) 9.67 ms per 100 sweeps,
-        // 10 x 8 chunks of 64 (256 tiles, the busiest-CU model's choice)
-        // 10.2, 12 x 8 of 64 (208) 9.88; the HBM bytes are the same.
-        if (chunk_req <= 0 && 2LL * c.nx * c.ny * c.nz * 4 >= (768LL << 20)) {
+        // 2/3 of the largest height or more, as measured; ties go to more
+        // tiles).  At 512^3: 8 waves x 5 chunks of 103 planes (215 tiles)
+        // 9.67 ms per 100 sweeps, 10 x 8 chunks of 64 (256 tiles, the
+        // busiest-CU model's choice) 10.2, 12 x 8 of 64 (208) 9.88, with the
+        // same HBM bytes; at 448^3 8 x 6 of 75 (228 tiles) 6.72 against 6.98.
+        // At 384^3 the busiest-CU model's 256 tiles stay the fastest.
+        if (chunk_req <= 0 && 2LL * c.nx * c.ny * c.nz * 4 >= (640LL << 20)) {
             SmPlan pick{};
             double pick_amp = 0;
             for (int wy = wmax; wy >= (wmax * 2) / 3 && wy >= wmin; --wy) {
@@ -639,7 +641,7 @@ st_status resolve_stream(Cfg &c)
                     const int ntiles = tiles_y * ((c.nz + chunk - 1) / chunk);
                     if (chunk < 4 || ntiles * 100 < cus * 75 || ntiles * 100 > cus * 92) continue;
                     const double amp = (chunk + 4.0) / chunk;
-                    if (pick.wy == 0 || amp < pick_amp - 1e-9) {
+                    if (pick.wy == 0 || amp < pick_amp - 1e-9 || (amp < pick_amp + 1e-9 && ntiles > pick.ntiles)) {
                         pick = stream_plan(c, wy, rpw, cus, chunk);
                         pick_amp = amp;
                     }
