@@ -2,8 +2,9 @@
 
 A fused pass of two sweeps that takes both levels' shell values from the
 buffer it reads differs from the per-sweep ping-pong only at interior cells
-with an index 1, 2, n-3 or n-2 on some axis (radius 1); the streaming
-kernel recomputes exactly those cells afterwards.  Checked here with the
+with an index 1, 2, n-3 or n-2 on some axis (radius 1); the band fix-up
+kernel k_stream_fix recomputes exactly those cells after each fused pass.
+Checked here with the
 oracle: the equal-shell run of the source buffer stands for the fused pass,
 the ping-pong with the true shells for the reference.
 """
