@@ -11,6 +11,9 @@ one JSON line.
              on the launch stream
   cpu_baseline  the CPU oracle (C, OpenMP) on a bounded sample of the same
              workload, on this host's cores
+  warm_start the metric's second half: time to relative residual 1e-6 at
+             128^3, learned warm start over plain zero start, on two sets of
+             five problems (--no-warm skips it)
 """
 from __future__ import annotations
 
