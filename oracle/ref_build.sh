@@ -21,7 +21,7 @@ CONFIGS=(
   "8 2 1 1" "8 2 1 2" "8 2 1 3" "16 2 1 10" "16 2 2 10" "16 2 3 7" "16 2 4 10"
   "33 2 1 12" "40 2 8 6" "40 2 16 4" "256 2 1 64"
   "8 3 1 1" "8 3 1 2" "8 3 1 3" "12 3 1 10" "12 3 2 10" "16 3 3 6" "16 3 4 10"
-  "20 3 8 4" "64 3 1 100" "48 3 4 20"
+  "20 3 8 4" "64 3 1 100" "48 3 4 20" "40 3 16 3" "36 3 16 2"
 )
 
 for cfg in "${CONFIGS[@]}"; do

@@ -78,12 +78,18 @@ CASES = [
     (2, 90, 80, 1, 4, 0, "f32", 5),
     (2, 70, 61, 1, 1, 1, "f64", 8),
     (2, 70, 61, 1, 16, 1, "f32", 3),
+    # 3D radius 16, as scripts/evaluate_stencils.py:11 sweeps it
+    (3, 40, 36, 34, 16, 1, "f32", 2),
+    (3, 38, 35, 34, 16, 0, "f32", 3),
+    (3, 36, 34, 33, 16, 1, "f64", 2),
 ]
 
 
 def _supported(variant, dims, r, dtype):
     if variant == "coop":
         return dims == 3 and r <= 4
+    if variant == "smem_padded" and dims == 3 and r > 8:
+        return False  # the (b+2R)^3 tile exceeds 160 KiB of LDS (test_padded_r16_3d_rejected)
     if variant == "smem_register":
         return r <= 8 and (dtype == "f32" or r in (1, 2, 4))
     return True
@@ -209,9 +215,30 @@ def test_residual_norm_gpu(gpu, oracle):
         assert abs(got - want) <= 1e-12 * want
 
 
+def test_padded_r16_3d_rejected(gpu):
+    """SMEM_PADDED's (bx+2R)(by+2R)(bz+2R) tile cannot fit 160 KiB at 3D R=16
+    for any block with bx, by, bz >= R (check_early_exit's rule): ST_ELDS."""
+    st = gpu
+    p = st.StencilParams.make(nx=40, ny=36, nz=34, dimensions=3, radius=16, iterations=2, op=1,
+                              variant="smem_padded", heuristic=1)
+    with pytest.raises(st.StencilError) as e:
+        st.check_early_exit(p)
+    assert e.value.status == 2
+
+
+MIXED_SHAPES = [
+    (3, (40, 33, 28), 1), (3, (36, 30, 26), 4), (2, (90, 70, 1), 2),
+    # rows over 256 columns: two waves of 4 cells per lane per row, the x
+    # neighbours across the wave seam exchanged through LDS (the layout the
+    # 384^3 workload runs)
+    (3, (300, 20, 18), 4), (3, (264, 16, 14), 1), (3, (384, 24, 20), 4), (3, (520, 14, 12), 2),
+    # nx not a multiple of 4: the per-cell 3D kernels
+    (3, (42, 33, 28), 1), (3, (42, 31, 27), 4),
+]
+
+
 @pytest.mark.parametrize("dtype", ["bf16", "f16"])
-@pytest.mark.parametrize("dims,shape,r", [(3, (40, 33, 28), 1), (3, (36, 30, 26), 4),
-                                          (2, (90, 70, 1), 2)])
+@pytest.mark.parametrize("dims,shape,r", MIXED_SHAPES, ids=str)
 def test_mixed_precision_vs_oracle(gpu, oracle, dtype, dims, shape, r):
     st = gpu
     nx, ny, nz = shape
@@ -276,6 +303,90 @@ def test_full_size_512(gpu, oracle):
         results[v] = hashlib.sha256(h.tobytes()).hexdigest()
         assert h.min() >= 0.0 and h.max() < 1.0
     assert len(set(results.values())) == 1, results
+
+
+@pytest.mark.parametrize("init", ["seeded", "reference"])
+def test_full_size_512_bench_schedule(gpu, oracle, init):
+    """The benchmark's own schedule at its own size: AUTO at 512^3 R=1 runs
+    the streaming form, one launch per fused pass of two sweeps (103-plane z
+    chunks, 215 tiles), checked against the oracle after 8 sweeps.  With the
+    reference's initial condition (buffer 2 zero, different shells) every
+    fused pass is followed by the band fix-up."""
+    st = gpu
+    n, its = 512, 8
+    op = st.OP_JACOBI if init == "seeded" else st.OP_THESIS
+    p = st.StencilParams.make(dim=n, radius=1, iterations=its, op=op, variant="auto", heuristic=1)
+    cfg = st.launch_config(p)
+    assert cfg["variant"] == st.V_COOP and cfg["temporal_steps"] == 2 and cfg["launches"] == its // 2
+    _announce("512^3 bench schedule", init, cfg)
+    d1 = st.DeviceArray(p.shape, np.float32)
+    d2 = st.DeviceArray(p.shape, np.float32)
+    st.initialize_region(p, d1, d2, st.INIT_SEEDED if init == "seeded" else st.INIT_REFERENCE,
+                         seed=0x5EED0002)
+    in_u2, _ = st.dispatch_kernels(p, d1, d2)
+    assert not in_u2
+    got = d1.numpy()
+    d1.free()
+    d2.free()
+    if init == "seeded":
+        u1 = oracle.init_seeded(n, n, n, 3, 1, 0x5EED0002)
+        u2 = u1.copy()
+    else:
+        u1 = oracle.init_reference(n, n, n, 3)
+        u2 = np.zeros_like(u1)
+    want = oracle.run(u1, u2, 3, 1, op, its)
+    assert_bitwise(got, want, f"512^3 bench schedule {init}")
+
+
+def test_full_size_mixed_384(gpu, oracle):
+    """The 384^3 R=4 workload at full size.  (1) Its 16-bit kernels (two
+    waves per row) bit-exact with the oracle over 23 sweeps (two defect
+    corrections).  (2) 200 sweeps per dtype: the FP64 field bit-exact with
+    the FP64 CPU oracle, and the FP32 / FP16+DC / BF16+DC fields within
+    stated max-abs bounds of it (measured 6.4e-7 / 6.0e-6 / 5.0e-5)."""
+    st = gpu
+    n, r, seed = 384, 4, 0x5EED0003
+    x0 = oracle.init_seeded(n, n, n, 3, r, seed)
+    shape = x0.shape
+
+    def mixed(dtype, its):
+        p = st.StencilParams.make(dim=n, dimensions=3, radius=r, iterations=its, op=1, dtype=dtype,
+                                  restart=10)
+        dx = st.DeviceArray.from_numpy(x0)
+        bufs = [st.DeviceArray(shape, np.uint16) for _ in range(3)]
+        du = st.DeviceArray(shape, np.float32)
+        st.dispatch_mixed(p, dx, *bufs, du)
+        out = du.numpy()
+        for b in [dx, du] + bufs:
+            b.free()
+        return out
+
+    for name, dt in (("bf16", st.BF16), ("f16", st.F16)):
+        _announce("384^3 mixed bit-exact", name)
+        want, folds = oracle.mixed_run(x0, 3, r, 23, 10, dt == st.F16)
+        assert folds == 2
+        assert_bitwise(mixed(dt, 23), want, f"384^3 mixed {name} 23 sweeps")
+    del want
+    its = 200
+    _announce("384^3 F64 x200")
+    x64 = oracle.init_seeded(n, n, n, 3, r, seed, dtype=np.float64)
+    ref = oracle.run(x64.copy(), x64.copy(), 3, r, 1, its).copy()
+    p = st.StencilParams.make(dim=n, dimensions=3, radius=r, iterations=its, op=1, variant="auto",
+                              dtype=st.F64)
+    got64, _ = run_gpu(st, p, x64, x64)
+    assert_bitwise(got64, ref, "384^3 R=4 F64 200 sweeps")
+    del got64, x64
+    bounds = {"f32": 2e-6, "f16": 2e-5, "bf16": 2e-4}
+    for name in ("f32", "f16", "bf16"):
+        _announce("384^3 x200 error", name)
+        if name == "f32":
+            q = p.copy(dtype=st.F32)
+            got, _ = run_gpu(st, q, x0, x0)
+        else:
+            got = mixed(st.F16 if name == "f16" else st.BF16, its)
+        err = np.abs(got.astype(np.float64) - ref)
+        print(f"[case] {name}: max-abs {err.max():.3e}, rms {np.sqrt(np.mean(err ** 2)):.3e}", flush=True)
+        assert err.max() <= bounds[name], (name, err.max())
 
 
 # register-queue temporal form (two fused sweeps per HBM pass)
@@ -377,10 +488,13 @@ def test_coop_stream_form(gpu, oracle, shape, knobs, op, init, per_pass):
         assert_bitwise(other[shell], orig[shell], f"stream form shell {shape} {knobs} op={op} {init} its={its}")
 
 
+@pytest.mark.parametrize("per_pass", [0, 1])
 @pytest.mark.parametrize("face", ["x0", "x1", "y0", "y1", "z0", "z1", "edge", "corner"])
-def test_coop_stream_one_shell_cell_differs(gpu, oracle, face):
-    """Buffers whose shells differ in one cell must take the per-sweep path
-    (the fused path would use u1's value for u2's shell)."""
+def test_coop_stream_one_shell_cell_differs(gpu, oracle, face, per_pass):
+    """Buffers whose shells differ in one cell: one cooperative launch must
+    take the per-sweep path (a fused pass would use u1's value for u2's
+    shell); one launch per pass (AUTO's and the benchmark's schedule) fuses
+    and then recomputes the band next to the shell."""
     st = gpu
     nx, ny, nz = 64, 33, 29
     u1 = oracle.init_seeded(nx, ny, nz, 3, 1, 0x5EED0007)
@@ -393,9 +507,10 @@ def test_coop_stream_one_shell_cell_differs(gpu, oracle, face):
     for its in (4, 6):
         want = oracle.run(u1.copy(), u2.copy(), 3, 1, 1, its).copy()
         p = st.StencilParams.make(nx=nx, ny=ny, nz=nz, dimensions=3, radius=1, iterations=its,
-                                  op=1, variant="coop", temporal_form=3, heuristic=1)
+                                  op=1, variant="coop", temporal_form=3, heuristic=1,
+                                  launch_per_pass=per_pass)
         got, _ = run_gpu(st, p, u1, u2)
-        assert_bitwise(got, want, f"shell cell {face} its={its}")
+        assert_bitwise(got, want, f"shell cell {face} its={its} per_pass={per_pass}")
 
 
 def test_coop_stream_form_rejects_unsupported(gpu):

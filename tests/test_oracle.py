@@ -267,6 +267,13 @@ def test_sse_closed_form(oracle):
     b[::10] = 0.5  # 100 cells differ by 0.5
     assert oracle.sse(a, b) == 25.0
     assert oracle.sse(a, a) == 0.0
+    # the square is rounded to FP32 before the FP64 sum
+    # (include/stencil_error_checker.h:15): (1 + 2^-12)^2 = 1 + 2^-11 + 2^-24
+    # is a tie in FP32 and rounds to even, 1 + 2^-11; squared in FP64 it
+    # would keep the 2^-24 term
+    d = np.float32(1 + 2.0 ** -12)
+    assert oracle.sse(np.array([d], np.float32), np.zeros(1, np.float32)) == 1 + 2.0 ** -11
+    assert (float(d) * float(d)) != 1 + 2.0 ** -11
 
 
 # ------------------------------------------------------------------ 16-bit storage

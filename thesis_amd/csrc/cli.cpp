@@ -13,11 +13,18 @@
 //     what the tuner parses;
 //   * --debug compares the final field with the solution file
 //     solutions/solution_<DIM>_<IT>_<R>_<D>d (include/stencil_utils.h:16-76):
-//     raw float32, x fastest, no header.  When the file is missing it is
-//     produced first by the BASE kernel family on the GPU (the reference
-//     runs its CPU loop here).  The check prints "rms error = %f" and fails
-//     when the sum of squared differences exceeds 1e-5
-//     (include/stencil_error_checker.h:6-20).
+//     raw values (float32, or float64 with --dtype f64), x fastest, no
+//     header.  That name is the reference's for its own problem (THESIS
+//     operator, FP32, the reference initial condition, a cubic grid); a run
+//     that changes any of these appends what differs (the extents, _jacobi,
+//     _f64, _seeded<seed>), so a cached solution is only ever compared with
+//     the problem that made it.  A file of the wrong length is refused.
+//     When the file is missing it is produced first by the BASE kernel
+//     family on the GPU (the reference runs its CPU loop stencil_cpu() here,
+//     src/main.cu:18-23); a CPU-made solution, such as the reference's own
+//     solution files, is checked when it is placed under that name.  The
+//     check prints "rms error = %f" and fails when the sum of squared
+//     differences exceeds 1e-5 (include/stencil_error_checker.h:6-20).
 #include <sys/stat.h>
 #include <sys/types.h>
 
@@ -103,13 +110,19 @@ int main(int argc, char **argv)
             else { usage(); return 1; }
         } else if (a == "--op") {
             std::string v = next();
-            p.op = v == "jacobi" ? ST_OP_JACOBI : ST_OP_THESIS;
+            if (v == "jacobi") p.op = ST_OP_JACOBI;
+            else if (v == "thesis") p.op = ST_OP_THESIS;
+            else { usage(); return 1; }
         } else if (a == "--dtype") {
             std::string v = next();
-            p.dtype = v == "f64" ? ST_F64 : ST_F32;
+            if (v == "f64") p.dtype = ST_F64;
+            else if (v == "f32") p.dtype = ST_F32;
+            else { usage(); return 1; }
         } else if (a == "--init") {
             std::string v = next();
-            init = v == "seeded" ? ST_INIT_SEEDED : ST_INIT_REFERENCE;
+            if (v == "seeded") init = ST_INIT_SEEDED;
+            else if (v == "reference") init = ST_INIT_REFERENCE;
+            else { usage(); return 1; }
         } else if (a == "-h" || a == "--help") {
             usage();
             return 0;
@@ -153,8 +166,23 @@ int main(int argc, char **argv)
     std::vector<unsigned char> host(n * es);
     if ((s = st_memcpy_dtoh(host.data(), in_u2 ? d2 : d1, n * es, nullptr)) != ST_OK) return die(s, "copy");
 
-    const std::string suffix = "_" + std::to_string(p.nx) + "_" + std::to_string(p.iterations) + "_" +
-                               std::to_string(p.radius) + "_" + std::to_string(p.dimensions) + "d";
+    // SUFFIX of include/stencil_utils.h:14, then whatever makes this problem
+    // differ from the reference's own
+    std::string suffix = "_" + std::to_string(p.nx) + "_" + std::to_string(p.iterations) + "_" +
+                         std::to_string(p.radius) + "_" + std::to_string(p.dimensions) + "d";
+    const bool cubic = p.ny == p.nx && (p.dimensions == 2 || p.nz == p.nx);
+    if (!cubic) {
+        suffix += "_" + std::to_string(p.nx) + "x" + std::to_string(p.ny);
+        if (p.dimensions == 3) suffix += "x" + std::to_string(p.nz);
+    }
+    if (p.op == ST_OP_JACOBI) suffix += "_jacobi";
+    if (p.dtype == ST_F64) suffix += "_f64";
+    if (init == ST_INIT_SEEDED) {
+        char sb[32];
+        std::snprintf(sb, sizeof sb, "_seeded%llx", seed);
+        suffix += sb;
+    }
+    const int nz_print = p.dimensions == 3 ? p.nz : 1;
     bool error = false;
     if (debug) {
         const std::string sol = "solutions/solution" + suffix;
@@ -176,33 +204,40 @@ int main(int argc, char **argv)
             mkdir("solutions", 0700);
             FILE *f = std::fopen(sol.c_str(), "wb");
             if (!f) { std::perror(sol.c_str()); return 1; }
-            std::fwrite(ref.data(), 1, ref.size(), f);
-            std::fclose(f);
-            std::printf("saving %s\n", sol.c_str());
+            const bool ok = std::fwrite(ref.data(), 1, ref.size(), f) == ref.size();
+            if (std::fclose(f) != 0 || !ok) { std::perror(sol.c_str()); return 1; }
+            std::printf("saving %s: nx = %d ny = %d nz = %d iterations: %d\n", sol.c_str(), p.nx, p.ny,
+                        nz_print, p.iterations);
         }
         std::vector<unsigned char> ref(n * es);
-        FILE *f = std::fopen(sol.c_str(), "rb");
+        struct stat sb;
+        FILE *f = nullptr;
+        const bool size_ok = stat(sol.c_str(), &sb) == 0 && (size_t)sb.st_size == ref.size();
+        if (size_ok) f = std::fopen(sol.c_str(), "rb");
         if (!f || std::fread(ref.data(), 1, ref.size(), f) != ref.size()) {
+            if (f) std::fclose(f);
             std::printf("Unable to load file: %s\n", sol.c_str());
             return 1;
         }
         std::fclose(f);
-        std::printf("reading %s\n", sol.c_str());
+        std::printf("reading %s: nx = %d ny = %d nz = %d\n", sol.c_str(), p.nx, p.ny, nz_print);
         double err = 0.0;
         for (size_t i = 0; i < n; ++i) {
-            double d;
             if (es == 4) {
+                // the FP32 difference squared in FP32, accumulated in FP64
+                // (include/stencil_error_checker.h:15)
                 float a, b;
                 std::memcpy(&a, &host[i * 4], 4);
                 std::memcpy(&b, &ref[i * 4], 4);
-                d = (double)(a - b);
+                const float d = a - b;
+                err += (double)(d * d);
             } else {
                 double a, b;
                 std::memcpy(&a, &host[i * 8], 8);
                 std::memcpy(&b, &ref[i * 8], 8);
-                d = a - b;
+                const double d = a - b;
+                err += d * d;
             }
-            err += d * d;
         }
         std::printf("rms error = %f \n", std::sqrt(err / (double)n));
         error = !(err <= 10e-6);
@@ -211,10 +246,11 @@ int main(int argc, char **argv)
         mkdir("results", 0700);
         const std::string res = "results/result" + suffix;
         FILE *f = std::fopen(res.c_str(), "wb");
-        if (f) {
-            std::fwrite(host.data(), 1, host.size(), f);
-            std::fclose(f);
-        }
+        if (!f) { std::perror(res.c_str()); return 1; }
+        const bool ok = std::fwrite(host.data(), 1, host.size(), f) == host.size();
+        if (std::fclose(f) != 0 || !ok) { std::perror(res.c_str()); return 1; }
+        std::printf("saving %s: nx = %d ny = %d nz = %d iterations: %d\n", res.c_str(), p.nx, p.ny, nz_print,
+                    p.iterations);
     }
     std::printf("%.4f\n", ms);
     st_free(d1);
