@@ -8,9 +8,17 @@ one JSON line.
   value      GCells/s = 510^3 interior cells x 100 sweeps x steps / time
   roofline   dominant kernel vs the 8 TB/s HBM peak, algorithmic bytes
              4 B x (512^3 + 510^3) per sweep, launch duration from HIP events
-             on the launch stream
+             on the launch stream; frac_per_hbm_pass divides the bytes one
+             HBM pass must move (the same figure, once per launch of two
+             fused sweeps) by the launch duration
   cpu_baseline  the CPU oracle (C, OpenMP) on a bounded sample of the same
              workload, on this host's cores
+  legs       the other measured workloads, each with its own roofline:
+             mixed384 (384^3 R=4, 200 sweeps in FP32, FP64, BF16 and FP16
+             storage with defect correction; errors against the FP64 CPU
+             oracle, whose run is that leg's CPU baseline) and batched (512
+             uneven grids, 50 sweeps, one graph launch per sweep against
+             per-problem launches); --no-legs skips them
   warm_start the metric's second half: time to relative residual 1e-6 at
              128^3, learned warm start over plain zero start, on two sets of
              five problems (--no-warm skips it)
@@ -51,11 +59,13 @@ def _cpu_model() -> str:
 def warm_start_leg(problems: int = 5, coarse_sweeps: int = 2000):
     """The metric's second half: wall time to relative residual 1e-6 on
     128^3 Laplace problems with Fourier-mode faces, learned warm start
-    (the tuned U-Net's 64^3 prediction, upsampled) against the plain zero
-    start, and the same with 2000 coarse JACOBI sweeps before the upsampling
-    (and that refinement of the network's input guess alone, the control).
-    Two sets of `problems` each: the five the network's training was judged
-    on, and five further ones never used in any choice."""
+    against the plain zero start.  The pipeline of record is
+    "learned_coarse": the tuned U-Net's 64^3 prediction refined by 2000
+    JACOBI sweeps at 64^3, then upsampled; "learned" (the prediction
+    upsampled directly) and "blend_coarse" (the same refinement of the
+    network's input guess, the control without the network) are reported
+    beside it.  Two sets of `problems` each: the five the network's training
+    was judged on, and five further ones never used in any choice."""
     import numpy as np
     from thesis_amd import warmstart
 
@@ -69,16 +79,23 @@ def warm_start_leg(problems: int = 5, coarse_sweeps: int = 2000):
                                              check_every=200, coarse_sweeps=coarse_sweeps)
             rows.append({"set": label, "plain_sweeps": r["plain"]["sweeps"],
                          "plain_s": round(r["plain"]["seconds"], 4), "learned_sweeps": r["learned"]["sweeps"],
+                         "learned_s": round(r["learned"]["seconds"], 4),
+                         "network_s": round(r["learned"]["network_seconds"], 5),
+                         "learned_coarse_sweeps": r["learned_coarse"]["sweeps"],
                          "learned_ratio": round(r["time_ratio_learned_vs_plain"], 4),
                          "learned_coarse_ratio": round(r["time_ratio_learned_coarse_vs_plain"], 4),
                          "blend_coarse_ratio": round(r["time_ratio_blend_coarse_vs_plain"], 4)})
     mean = lambda k, sel=None: round(float(np.mean([r[k] for r in rows if sel is None or r["set"] == sel])), 4)
     return {"metric": "time to relative residual 1e-6 at 128^3: learned warm start / plain zero start",
             "unit": "ratio", "higher_is_better": False, "problems": len(rows),
+            "of_record": "learned_coarse",
+            "value": mean("learned_coarse_ratio"),
+            "value_max": round(max(r["learned_coarse_ratio"] for r in rows), 4),
+            "learned_coarse": mean("learned_coarse_ratio"), "blend_coarse": mean("blend_coarse_ratio"),
             "learned": mean("learned_ratio"), "learned_standard": mean("learned_ratio", "standard"),
             "learned_further": mean("learned_ratio", "further"),
             "learned_max": round(max(r["learned_ratio"] for r in rows), 4),
-            "learned_coarse": mean("learned_coarse_ratio"), "blend_coarse": mean("blend_coarse_ratio"),
+            "network_seconds": mean("network_s"),
             "coarse_sweeps": coarse_sweeps, "per_problem": rows}
 
 
@@ -117,17 +134,29 @@ def cpu_baseline(budget_s: float = 12.0, single_budget_s: float = 6.0):
 
 
 def load_traffic(tag: str):
+    """HBM bytes per launch of the dominant kernel from a committed
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary (calibrated against the
+    copy kernel), with the file it came from."""
     path = os.path.join(ROOT, "profiles", f"traffic_{tag}.json")
     if os.path.exists(path):
         with open(path) as f:
-            return json.load(f).get("bytes_per_launch")
-    return None
+            d = json.load(f)
+        return d.get("bytes_per_launch"), f"profiles/traffic_{tag}.json ({d.get('kernel', '?')}: " \
+                                          f"{d.get('workload', '?')})"
+    return None, None
 
 
-def bench_batched(st, np, args):
+def _median(xs):
+    xs = sorted(xs)
+    return xs[len(xs) // 2] if len(xs) % 2 else 0.5 * (xs[len(xs) // 2 - 1] + xs[len(xs) // 2])
+
+
+def batched_leg(st, np, repeats: int = 3):
     """512 uneven 3D grids (edges log-uniform in [16, 128], radius 1 or 2),
-    50 JACOBI sweeps each: one hipGraph launch, one persistent cooperative
-    launch, and per-problem launches."""
+    50 JACOBI sweeps each: one hipGraph launch per sweep over all grids, one
+    persistent cooperative launch, and per-problem launches.  The roofline
+    is the graph mode's: algorithmic bytes of one sweep of every grid
+    (4 B x (cells + interior cells)) over the time per sweep."""
     from thesis_amd import workloads
     from thesis_amd.tuner.batched_tiles import load_rules
 
@@ -144,77 +173,143 @@ def bench_batched(st, np, args):
         probs.append(dict(nx=nx, ny=ny, nz=nz, radius=r, iterations=its, op=st.OP_JACOBI, u1=d1, u2=d2))
         bufs += [d1, d2]
     stream.sync()
-    cells = workloads.batched_interior_cells(shapes) * its
-    traffic = 4 * (workloads.batched_cells(shapes) + workloads.batched_interior_cells(shapes)) * its
+    interior = workloads.batched_interior_cells(shapes)
+    bytes_sweep = 4 * (workloads.batched_cells(shapes) + interior)
     res = {}
     for name, mode in (("graph", st.BATCH_GRAPH), ("persistent", st.BATCH_PERSISTENT),
                        ("sequential", st.BATCH_SEQUENTIAL)):
-        st.dispatch_batched(probs, rules, mode, stream=stream)  # warm-up
-        t = min(st.dispatch_batched(probs, rules, mode, stream=stream, timed=True)
-                for _ in range(max(1, args.steps)))
-        res[name] = {"ms": round(t, 3), "gcells_per_s": round(cells / t / 1e6, 2),
-                     "hbm_frac": round(traffic / (t / 1e3) / 8e12, 4)}
-    out = {"metric": "batched GCells/s (512 uneven grids, 50 sweeps)", "value": res["graph"]["gcells_per_s"],
-           "unit": "GCells/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
-           "data": "synthetic: 512 grids, edges round(16*8^U) per axis, radius 1|2, seeded boundaries",
-           "config": {"workload": "batched uneven 3D JACOBI, 50 sweeps", "total_cells": workloads.batched_cells(shapes),
-                      "tile_rules": rules},
-           "modes": res,
-           "speedup_graph_vs_sequential": round(res["sequential"]["ms"] / res["graph"]["ms"], 2),
-           "speedup_persistent_vs_sequential": round(res["sequential"]["ms"] / res["persistent"]["ms"], 2)}
-    print(json.dumps(out), flush=True)
+        # the first dispatch in graph mode captures and instantiates the
+        # graph (counted in its time); later ones replay it
+        first = st.dispatch_batched(probs, rules, mode, stream=stream, timed=True)
+        ts = [st.dispatch_batched(probs, rules, mode, stream=stream, timed=True) for _ in range(repeats)]
+        t = _median(ts)
+        res[name] = {"ms": round(t, 3), "gcells_per_s": round(interior * its / t / 1e6, 2),
+                     "hbm_frac": round(bytes_sweep * its / (t / 1e3) / 8e12, 4),
+                     "first_call_ms": round(first, 3)}
+    for b in bufs:
+        b.free()
+    per_sweep_s = res["graph"]["ms"] / 1e3 / its
+    achieved = bytes_sweep / per_sweep_s / 1e9
+    return {"metric": "batched GCells/s (512 uneven grids, 50 sweeps)", "value": res["graph"]["gcells_per_s"],
+            "unit": "GCells/s", "higher_is_better": True, "dtype": "f32",
+            "data": "synthetic: 512 grids, edges round(16*8^U) per axis, radius 1|2, seeded boundaries",
+            "config": {"workload": "batched uneven 3D JACOBI, 50 sweeps", "total_cells": workloads.batched_cells(shapes),
+                       "tile_rules": rules},
+            "modes": res,
+            "speedup_graph_vs_sequential": round(res["sequential"]["ms"] / res["graph"]["ms"], 2),
+            "speedup_persistent_vs_sequential": round(res["sequential"]["ms"] / res["persistent"]["ms"], 2),
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                         "bytes_per_launch_algorithmic": bytes_sweep,
+                         "avg_launch_ms": round(per_sweep_s * 1e3, 5),
+                         "note": "one launch is one sweep of all 512 grids (graph mode, replaying the "
+                                 "instantiated graph; the time per sweep includes the shell comparison, one "
+                                 "launch per dispatch; modes.graph.first_call_ms adds capture and "
+                                 "instantiation)"}}
 
 
-def bench_mixed(st, np, args):
-    """384^3, radius 4 (25-point), JACOBI c_r = 1/24, 200 sweeps: FP32, FP64 and
-    BF16/FP16 storage with FP32 defect correction every 10 sweeps.  Errors are
-    measured against the FP64 field on the GPU (whose bit-exactness with the CPU
-    FP64 oracle the parity tests establish)."""
+def mixed_leg(st, np, repeats: int = 3, with_cpu: bool = True):
+    """384^3, radius 4 (25-point), JACOBI c_r = 1/24, 200 sweeps: FP32, FP64,
+    and BF16 / FP16 storage with FP32 arithmetic and FP32 defect correction
+    every 10 sweeps.  Errors (max-abs, relative L2) are taken against the
+    FP64 CPU oracle run of the same 200 sweeps, which is also this leg's CPU
+    baseline.  The roofline is the BF16 correction sweep's, the leg's
+    dominant kernel: per-sweep algorithmic bytes 2 B x (cells + interior
+    cells), plus the defect read at every interior cell in the
+    kernel-bytes figure; its time per sweep is (t(200) - t(100)) / 100 of
+    dispatches without defect correction, which cancels the fixed passes."""
     n, r, its = 384, 4, 200
     seed = 0x5EED0003
-    base = dict(dim=n, radius=r, iterations=its, op=st.OP_JACOBI, variant="auto", heuristic=1)
+    base = dict(dim=n, radius=r, iterations=its, op=st.OP_JACOBI, heuristic=1)
     shape = (n, n, n)
-    interior = (n - 2 * r) ** 3
-    out = {"metric": "384^3 R=4 error vs FP64 oracle per storage type", "unit": "GCells/s",
-           "config": {"workload": "3D 25-point JACOBI, 384^3, 200 sweeps, restart 10"}, "results": {}}
-    p64 = st.StencilParams.make(dtype=st.F64, **base)
-    a = st.DeviceArray(shape, np.float64)
-    b = st.DeviceArray(shape, np.float64)
-    st.initialize_region(p64, a, b, st.INIT_SEEDED, seed=seed)
-    _, ms = st.dispatch_kernels(p64, a, b, timed=True)
-    ref = a.numpy()
-    a.free(); b.free()
-    out["results"]["f64"] = {"ms": round(ms, 3), "gcells_per_s": round(interior * its / ms / 1e6, 1),
-                             "max_abs": 0.0, "l2": 0.0}
+    cells, interior = n ** 3, (n - 2 * r) ** 3
+    out = {"metric": "GCells/s and error vs the FP64 CPU oracle per storage type (384^3 r=4, 200 sweeps)",
+           "unit": "GCells/s", "higher_is_better": True,
+           "config": {"workload": "3D 25-point JACOBI u'=(1/24)*sum of 24 neighbours, 384^3, radius 4, "
+                                  "200 sweeps; 16-bit storage with FP32 defect correction every 10 sweeps"},
+           "data": "synthetic: seeded uniform [0,1) boundary shell (4 cells thick), zero interior",
+           "results": {}}
+    ref = None
+    if with_cpu:
+        from oracle import oracle as O
+        x64 = O.init_seeded(n, n, n, 3, r, seed, dtype=np.float64)
+        t0 = time.perf_counter()
+        ref = O.run(x64, x64.copy(), 3, r, O.OP_JACOBI, its).copy()
+        el = time.perf_counter() - t0
+        threads = O.num_threads()
+        out["cpu_baseline"] = {"value": round(interior * its / el / 1e9, 4), "unit": "GCells/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"the whole leg in FP64: {its} sweeps of the {n}^3 R={r} JACOBI field "
+                                         f"({el:.1f} s, oracle/stencil_oracle.c, OpenMP {threads} threads); "
+                                         f"its field is the error reference"}
+        del x64
 
-    def errs(u):
-        d = u.astype(np.float64) - ref
-        return float(np.max(np.abs(d))), float(np.sqrt(np.sum(d * d)) / np.sqrt(np.sum(ref * ref)))
+    def rec(ms, es_bytes, u=None):
+        d = {"ms": round(ms, 3), "gcells_per_s": round(interior * its / ms / 1e6, 1),
+             "hbm_frac": round(es_bytes * (cells + interior) * its / (ms / 1e3) / 8e12, 4)}
+        if u is not None and ref is not None:
+            diff = u.astype(np.float64) - ref
+            d["max_abs"] = float(np.max(np.abs(diff)))
+            d["l2_rel"] = float(np.sqrt(np.sum(diff * diff)) / np.sqrt(np.sum(ref * ref)))
+            d["bitwise_equal_to_oracle"] = bool(u.dtype == np.float64 and np.array_equal(
+                u.view(np.uint64), ref.view(np.uint64)))
+        return d
 
-    p32 = st.StencilParams.make(dtype=st.F32, **base)
-    a = st.DeviceArray(shape, np.float32)
-    b = st.DeviceArray(shape, np.float32)
-    st.initialize_region(p32, a, b, st.INIT_SEEDED, seed=seed)
-    x0 = a.numpy()
-    _, ms = st.dispatch_kernels(p32, a, b, timed=True)
-    mx, l2 = errs(a.numpy())
-    out["results"]["f32"] = {"ms": round(ms, 3), "gcells_per_s": round(interior * its / ms / 1e6, 1),
-                             "max_abs": mx, "l2_rel": l2}
+    for name, dt, npdt, es in (("f64", st.F64, np.float64, 8), ("f32", st.F32, np.float32, 4)):
+        p = st.StencilParams.make(dtype=dt, variant="auto", **base)
+        a = st.DeviceArray(shape, npdt)
+        b = st.DeviceArray(shape, npdt)
+        ts = []
+        for _ in range(repeats + 1):
+            st.initialize_region(p, a, b, st.INIT_SEEDED, seed=seed)
+            _, ms = st.dispatch_kernels(p, a, b, timed=True)
+            ts.append(ms)
+        out["results"][name] = rec(_median(ts[1:]), es, a.numpy())
+        if name == "f32":  # the initial field of the 16-bit runs' FP32 base
+            st.initialize_region(p, a, b, st.INIT_SEEDED, seed=seed)
+            x0 = a.numpy()
+        a.free()
+        b.free()
+    bufs = {}
     for name, dt in (("bf16", st.BF16), ("f16", st.F16)):
-        pm = st.StencilParams.make(dtype=dt, restart=10, **{k: v for k, v in base.items() if k != "variant"})
-        x = st.DeviceArray.from_numpy(x0)
-        e1 = st.DeviceArray(shape, np.uint16)
-        e2 = st.DeviceArray(shape, np.uint16)
-        d = st.DeviceArray(shape, np.uint16)
+        pm = st.StencilParams.make(dtype=dt, restart=10, **base)
+        x = st.DeviceArray(shape, np.float32)
+        e1, e2, d = (st.DeviceArray(shape, np.uint16) for _ in range(3))
         u = st.DeviceArray(shape, np.float32)
-        ms = st.dispatch_mixed(pm, x, e1, e2, d, u, timed=True)
-        mx, l2 = errs(u.numpy())
-        out["results"][name] = {"ms": round(ms, 3), "gcells_per_s": round(interior * its / ms / 1e6, 1),
-                                "max_abs": mx, "l2_rel": l2}
-        for t in (x, e1, e2, d, u):
-            t.free()
-    out["value"] = out["results"]["f32"]["gcells_per_s"]
-    print(json.dumps(out), flush=True)
+        ts = []
+        for _ in range(repeats + 1):
+            x.upload(x0)
+            ts.append(st.dispatch_mixed(pm, x, e1, e2, d, u, timed=True))
+        out["results"][name] = rec(_median(ts[1:]), 2, u.numpy())
+        out["results"][name]["restart"] = 10
+        bufs[name] = (pm, x, e1, e2, d, u)
+    # the BF16 correction sweep alone: dispatches with no defect correction
+    pm, x, e1, e2, d, u = bufs["bf16"]
+    per = []
+    for _ in range(repeats):
+        tt = {}
+        for k in (200, 100):
+            q = pm.copy(iterations=k, restart=k)
+            x.upload(x0)
+            tt[k] = st.dispatch_mixed(q, x, e1, e2, d, u, timed=True)
+        per.append((tt[200] - tt[100]) / 100.0)
+    sweep_ms = _median(per)
+    alg = 2 * (cells + interior)
+    kern = alg + 2 * interior
+    achieved = alg / (sweep_ms / 1e3) / 1e9
+    out["roofline"] = {"bound": "hbm", "kernel": "BF16 correction sweep (row kernel, 16-bit mode)",
+                       "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                       "bytes_per_launch_algorithmic": alg, "bytes_per_launch_kernel": kern,
+                       "frac_kernel_bytes": round(kern / (sweep_ms / 1e3) / 8e12, 4),
+                       "avg_launch_ms": round(sweep_ms, 5),
+                       "note": "algorithmic bytes 2 B x (cells + interior) per sweep; the kernel also reads "
+                               "the 16-bit defect at every interior cell (bytes_per_launch_kernel)"}
+    for t in bufs.values():
+        for b in t[1:]:
+            b.free()
+    out["value"] = out["results"]["bf16"]["gcells_per_s"]
+    return out
 
 
 def probe(st, np):
@@ -234,74 +329,17 @@ def probe(st, np):
         print(f"[probe] {v} form {form}: {its} sweeps, launch {st.launch_config(q)}", flush=True)
 
 
-def main():
-    global N, RADIUS, SWEEPS
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--variant", default="auto")
-    ap.add_argument("--temporal-steps", type=int, default=0)
-    ap.add_argument("--block", default="", help="bx,by,bz override (heuristic otherwise)")
-    ap.add_argument("--coarsen", default="", help="cx,cy override")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-warm", action="store_true",
-                    help="skip the learned warm-start leg of the headline line")
-    ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--workload", default="sweep512",
-                    choices=["sweep512", "batched", "mixed384"],
-                    help="sweep512: the headline 512^3 sweep (default); batched: 512 uneven "
-                         "grids in one graph / persistent launch vs per-problem launches; "
-                         "mixed384: 384^3 R=4 in FP32, FP64 and 16-bit storage with errors "
-                         "against the FP64 oracle")
-    ap.add_argument("--probe", action="store_true",
-                    help="short run for counter collection: one streaming copy of a 512^3 "
-                         "field, 2 sweeps of each single-sweep family and of the LDS-ring "
-                         "COOP form, then 100 sweeps of the streaming COOP form with one "
-                         "launch per pass (the headline's configuration)")
-    ap.add_argument("--dim", type=int, default=N, help="grid edge (default: the 512^3 headline)")
-    ap.add_argument("--radius", type=int, default=RADIUS)
-    ap.add_argument("--sweeps", type=int, default=SWEEPS, help="sweeps per step (even)")
-    ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
-    ap.add_argument("--init", default="seeded", choices=["seeded", "reference"],
-                    help="seeded: the same boundary in both buffers (two sweeps fuse per pass "
-                         "in the streaming form); reference: u2 zero (different shells, one "
-                         "sweep per pass)")
-    args = ap.parse_args()
-
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU slab halo exchange "
-                                   "is a design note only, so --gpus must be 1"}))
-        sys.exit(2)
-
-    import numpy as np
-
-    import thesis_amd as st
-
-    if st.device_count() < 1:
-        print(json.dumps({"error": "no GPU visible"}))
-        sys.exit(1)
-
-    if args.probe:
-        probe(st, np)
-        return
-    if args.workload == "batched":
-        bench_batched(st, np, args)
-        return
-    if args.workload == "mixed384":
-        bench_mixed(st, np, args)
-        return
-
+def headline_leg(st, np, args):
+    """The 512^3 sweep (or the grid the --dim/--radius/--sweeps/--dtype
+    knobs name): K timed steps of `sweeps` sweeps each after W warm-up
+    steps, synchronised on both sides."""
     headline = (args.dim, args.radius, args.sweeps, args.dtype) == (N, RADIUS, SWEEPS, "f32")
-    # the committed traffic profiles were measured on the seeded field (equal
-    # shells: fused sweeps, whole-row stores)
-    measured_traffic = headline and args.init == "seeded"
-    if args.sweeps % 2:
+    n, radius, sweeps = args.dim, args.radius, args.sweeps
+    if sweeps % 2:
         raise SystemExit("--sweeps must be even (the state stays in the first buffer)")
-    N, RADIUS, SWEEPS = args.dim, args.radius, args.sweeps
     f64 = args.dtype == "f64"
     es = 8 if f64 else 4
-    kw = dict(dim=N, radius=RADIUS, iterations=SWEEPS, op=st.OP_JACOBI, dtype=st.F64 if f64 else st.F32,
+    kw = dict(dim=n, radius=radius, iterations=sweeps, op=st.OP_JACOBI, dtype=st.F64 if f64 else st.F32,
               variant=args.variant, heuristic=1, temporal_steps=args.temporal_steps)
     if args.block:
         bx, by, bz = (int(v) for v in args.block.split(","))
@@ -315,7 +353,7 @@ def main():
     d1 = st.DeviceArray(p.shape, np.float64 if f64 else np.float32)
     d2 = st.DeviceArray(p.shape, np.float64 if f64 else np.float32)
 
-    # The field is initialised once; every step advances it by 100 more
+    # The field is initialised once; every step advances it by `sweeps` more
     # sweeps (an even count, so the state stays in d1).  The work per sweep
     # does not depend on the values.
     st.initialize_region(p, d1, d2, st.INIT_SEEDED if args.init == "seeded" else st.INIT_REFERENCE,
@@ -337,17 +375,39 @@ def main():
     wall = time.perf_counter() - t0
 
     sweep_total_s = sum(sweep_ms) / 1e3
-    cells = (N - 2 * RADIUS) ** 3 * SWEEPS * args.steps
+    cells = (n - 2 * radius) ** 3 * sweeps * args.steps
     gcells = cells / wall / 1e9
     launches = cfg["launches"]
-    bytes_sweep = algorithmic_bytes_per_sweep(N, RADIUS, es)
-    bytes_launch = bytes_sweep * SWEEPS / launches
+    bytes_sweep = algorithmic_bytes_per_sweep(n, radius, es)
+    bytes_launch = bytes_sweep * sweeps / launches
     avg_launch_s = sweep_total_s / (launches * args.steps)
     achieved = bytes_launch / avg_launch_s / 1e9
     variant_name = {v: k for k, v in st.VARIANTS.items()}[cfg["variant"]]
+    # the committed traffic summaries were measured on the seeded 512^3 field
+    traffic, traffic_src = load_traffic(variant_name) if headline and args.init == "seeded" else (None, None)
+    fused = variant_name == "coop" and cfg["temporal_steps"] > 1
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_launch_algorithmic": bytes_launch,
+            "avg_launch_ms": round(avg_launch_s * 1e3, 5)}
+    if traffic_src:
+        roof["traffic_source"] = traffic_src
+    if fused:
+        # one launch is one HBM pass that advances temporal_steps sweeps: the
+        # bytes a pass must move are one sweep's (read the grid, write the
+        # interior), so this is the fraction of the HBM peak the pass uses
+        roof["bytes_per_hbm_pass"] = bytes_sweep
+        roof["frac_per_hbm_pass"] = round(bytes_sweep / avg_launch_s / 1e9 / HBM_PEAK_GBS, 4)
+        roof["note"] = (f"a launch is one HBM pass of {cfg['temporal_steps']} fused sweeps, so `frac` (the "
+                        f"per-sweep algorithmic convention) can exceed 1; frac_per_hbm_pass is the physical "
+                        f"fraction; the event time per launch includes the shell comparison and the band "
+                        f"fix-up launches (which return at once with equal shells)")
+    else:
+        roof["note"] = "one sweep per launch"
     out = {
         "metric": "GCells/s & %HBM roofline (512^3 r=1)" if headline
-                  else f"GCells/s & %HBM roofline ({N}^3 r={RADIUS} {args.dtype})",
+                  else f"GCells/s & %HBM roofline ({n}^3 r={radius} {args.dtype})",
         "value": round(gcells, 3),
         "unit": "GCells/s",
         "n_gpus": 1,
@@ -361,24 +421,84 @@ def main():
         "dtype": args.dtype,
         "data": ("synthetic: seeded uniform [0,1) boundary shell, zero interior" if args.init == "seeded"
                  else "synthetic: reference initial condition (u1 shell 1, u2 zero)"),
-        "config": {"workload": (f"3D {6 * RADIUS + 1}-point JACOBI u'=(1/{6 * RADIUS})*sum of "
-                                f"{6 * RADIUS} neighbours, {N}^3 {args.dtype.upper()}, radius {RADIUS}, "
-                                f"{SWEEPS} sweeps per step"),
+        "config": {"workload": (f"3D {6 * radius + 1}-point JACOBI u'=(1/{6 * radius})*sum of "
+                                f"{6 * radius} neighbours, {n}^3 {args.dtype.upper()}, radius {radius}, "
+                                f"{sweeps} sweeps per step"),
                    "variant": variant_name, "launch": cfg, "device": st.device_name(0)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic(variant_name) if measured_traffic else None,
-                     "bytes_per_launch_algorithmic": bytes_launch,
-                     "avg_launch_ms": round(avg_launch_s * 1e3, 5),
-                     "note": ("temporal-blocked passes advance two sweeps per HBM pass, so the "
-                              "algorithmic fraction can exceed 1; a launch is one pass, and the "
-                              "event time per launch includes the shell comparison and the band "
-                              "fix-up launches (which return at once with equal shells)")
-                             if variant_name == "coop"
-                             else "one sweep per launch"},
+        "roofline": roof,
     }
     d1.free()
     d2.free()
+    stream.close()
+    return out, headline
+
+
+def main():
+    global N, RADIUS, SWEEPS
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--variant", default="auto")
+    ap.add_argument("--temporal-steps", type=int, default=0)
+    ap.add_argument("--block", default="", help="bx,by,bz override (heuristic otherwise)")
+    ap.add_argument("--coarsen", default="", help="cx,cy override")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-warm", action="store_true",
+                    help="skip the learned warm-start leg of the headline line")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="skip the mixed384 and batched legs of the headline line")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--workload", default="sweep512",
+                    choices=["sweep512", "batched", "mixed384"],
+                    help="sweep512: the headline 512^3 sweep with its legs (default); batched / "
+                         "mixed384: that leg alone, as its own line")
+    ap.add_argument("--probe", action="store_true",
+                    help="short run for counter collection: one streaming copy of a 512^3 "
+                         "field, 2 sweeps of each single-sweep family and of the LDS-ring "
+                         "COOP form, then 100 sweeps of the streaming COOP form with one "
+                         "launch per pass (the headline's configuration)")
+    ap.add_argument("--dim", type=int, default=N, help="grid edge (default: the 512^3 headline)")
+    ap.add_argument("--radius", type=int, default=RADIUS)
+    ap.add_argument("--sweeps", type=int, default=SWEEPS, help="sweeps per step (even)")
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--init", default="seeded", choices=["seeded", "reference"],
+                    help="seeded: the same boundary in both buffers; reference: u2 zero "
+                         "(different shells: the streaming form's band fix-up runs after each pass)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: the multi-GPU slab halo exchange is a "
+                                   "design note only, so --gpus must be 1"}))
+        sys.exit(2)
+
+    import numpy as np
+
+    import thesis_amd as st
+
+    if st.device_count() < 1:
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(1)
+
+    if args.probe:
+        probe(st, np)
+        return
+    if args.workload == "batched":
+        out = batched_leg(st, np, max(1, min(args.steps, 5)))
+        out["n_gpus"] = 1
+        print(json.dumps(out), flush=True)
+        return
+    if args.workload == "mixed384":
+        out = mixed_leg(st, np, max(1, min(args.steps, 5)), with_cpu=not args.no_cpu)
+        out["n_gpus"] = 1
+        print(json.dumps(out), flush=True)
+        return
+
+    out, headline = headline_leg(st, np, args)
+    if headline and not args.no_legs:
+        reps = max(1, min(args.steps, 3))
+        out["legs"] = {"mixed384": mixed_leg(st, np, reps, with_cpu=not args.no_cpu),
+                       "batched": batched_leg(st, np, reps)}
     if headline and not args.no_warm:
         out["warm_start"] = warm_start_leg()
     if not args.no_cpu:

@@ -256,7 +256,11 @@ typedef enum st_batch_mode {
  * when it is NULL).  elapsed_ms (optional) covers the launches on the
  * device: in the plane-chunk form the comparison of each problem's two
  * shells (equal shells let whole rows be stored, shell cells with the value
- * they hold) and the sweeps. */
+ * they hold) and the sweeps.  ST_BATCH_GRAPH keeps the instantiated graph of
+ * the last batch per device and stream and replays it when the next
+ * dispatch has the same problems (buffers, shapes, iteration counts) and
+ * tile rules; on the dispatch that captures and instantiates it, elapsed_ms
+ * includes that host work (the stream waits meanwhile). */
 st_status stencil_dispatch_batched(const st_problem *problems, int nproblems,
                                    const st_tile_rule *rules, int nrules, int mode,
                                    void *stream, float *elapsed_ms);

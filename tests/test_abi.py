@@ -84,7 +84,7 @@ def test_invalid_knobs(st):
     _raises(st, 1, dtype=st.BF16)                # 16-bit storage only through the mixed path
     _raises(st, 2, variant="smem_padded", block_x=16, block_y=8, block_z=8, radius=8, dim=64,
             coarsen_x=4)                         # LDS over 160 KiB
-    _raises(st, 5, variant="smem_register", radius=9, dim=64, heuristic=1)
+    _raises(st, 5, variant="smem_register", radius=17, dim=64, heuristic=1)
     _raises(st, 5, variant="coop", radius=5, dim=64, heuristic=1)
     _raises(st, 5, variant="coop", dimensions=2, dim=64, heuristic=1)
 

@@ -28,7 +28,7 @@ COMMON = [
 DEVICE = [
     f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-munsafe-fp-atomics",
+    "-munsafe-fp-atomics", "-Wno-pass-failed",
 ]
 SOURCES = [
     "dispatch.cpp",

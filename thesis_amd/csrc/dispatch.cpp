@@ -56,12 +56,27 @@ struct WsBlock {
 };
 static std::map<std::tuple<int, hipStream_t, int>, WsBlock> g_ws;
 
+// While set on this thread, workspace requests are served from a fixed
+// device buffer (one part per slot) and nothing is allocated.
+static thread_local const WsFixed *g_ws_fixed = nullptr;
+
+void workspace_use_fixed(const WsFixed *fixed) { g_ws_fixed = fixed; }
+bool workspace_is_fixed() { return g_ws_fixed != nullptr; }
+
 st_status workspace_slot(int slot, size_t bytes, hipStream_t stream, void **ptr)
 {
+    if (slot < 0 || slot >= kSlots) return fail(ST_EINVAL, "workspace slot %d", slot);
+    if (g_ws_fixed) {
+        const size_t part = g_ws_fixed->bytes[slot];
+        if (bytes > part)
+            return fail(ST_EINVAL, "workspace of %zu bytes exceeds the fixed scratch part of %zu bytes (slot %d)",
+                        bytes, part, slot);
+        *ptr = static_cast<unsigned char *>(g_ws_fixed->base) + g_ws_fixed->offset[slot];
+        return ST_OK;
+    }
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return hip_check(e, "hipGetDevice");
-    if (slot < 0 || slot >= kSlots) return fail(ST_EINVAL, "workspace slot %d", slot);
     std::lock_guard<std::mutex> lock(g_ws_mutex);
     WsBlock &w = g_ws[std::make_tuple(dev, stream, slot)];
     if (w.size < bytes) {

@@ -92,6 +92,19 @@ st_status run_mixed(const Cfg &c, void *x, void *e1, void *e2, void *d, void *u,
 // Library-owned device workspace (grows on demand, one per device, stream
 // and slot).
 st_status workspace_slot(int slot, size_t bytes, hipStream_t stream, void **ptr);
+// A fixed device buffer that serves every workspace request of the calling
+// thread while it is set (one part per slot; a request larger than its
+// slot's part fails with ST_EINVAL); null returns to the allocated
+// workspace.
+struct WsFixed {
+    void *base;
+    size_t offset[4];  // part of each slot: [offset, offset + bytes)
+    size_t bytes[4];
+};
+void workspace_use_fixed(const WsFixed *fixed);
+// True while a fixed buffer serves this thread's workspace (the no-alloc
+// entry point, whose callers may be capturing a graph).
+bool workspace_is_fixed();
 inline st_status workspace(size_t bytes, hipStream_t stream, void **ptr) { return workspace_slot(0, bytes, stream, ptr); }
 
 st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules, int nrules,

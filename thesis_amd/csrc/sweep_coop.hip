@@ -328,14 +328,22 @@ st_status run_coop(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2)
     float kf = (float)kcd;
     double kd = kcd;
     void *karg = c.dtype == ST_F64 ? (void *)&kd : (void *)&kf;
-    const bool coop = !c.per_pass;
+    bool coop = !c.per_pass;
     if (coop) {
         a.pass0 = 0;
         a.pass1 = a.npass;
         void *args[] = {&u1, &u2, &a, karg};
         e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, c.lds, s);
-        if (e != hipSuccess) return hip_check(e, "hipLaunchCooperativeKernel");
-    } else {
+        if (e == hipErrorIllegalState) {
+            // the device takes no cooperative launch from this process now:
+            // the same passes, one launch each
+            (void)hipGetLastError();
+            coop = false;
+        } else if (e != hipSuccess) {
+            return hip_check(e, "hipLaunchCooperativeKernel");
+        }
+    }
+    if (!coop) {
         for (int p = 0; p < a.npass; ++p) {
             a.pass0 = p;
             a.pass1 = p + 1;

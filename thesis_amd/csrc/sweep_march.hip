@@ -84,18 +84,30 @@ static const MarchEntry kMarch[] = {
     ME(3, float, 4, 1, 1), ME(3, float, 4, 1, 2), ME(3, float, 4, 4, 1), ME(3, float, 4, 4, 2),
     ME(3, float, 5, 1, 1), ME(3, float, 5, 1, 2), ME(3, float, 6, 1, 1), ME(3, float, 6, 1, 2),
     ME(3, float, 7, 1, 1), ME(3, float, 7, 1, 2), ME(3, float, 8, 1, 1), ME(3, float, 8, 1, 2),
+    ME(3, float, 9, 1, 1), ME(3, float, 10, 1, 1), ME(3, float, 11, 1, 1), ME(3, float, 12, 1, 1),
+    ME(3, float, 13, 1, 1), ME(3, float, 14, 1, 1), ME(3, float, 15, 1, 1), ME(3, float, 16, 1, 1),
     // 3D double
     ME(3, double, 1, 1, 1), ME(3, double, 1, 1, 2), ME(3, double, 1, 1, 4),
     ME(3, double, 1, 2, 1), ME(3, double, 1, 2, 2), ME(3, double, 1, 2, 4),
     ME(3, double, 2, 1, 1), ME(3, double, 2, 2, 1), ME(3, double, 2, 2, 2),
     ME(3, double, 4, 1, 1), ME(3, double, 4, 2, 1), ME(3, double, 4, 2, 2),
+    ME(3, double, 3, 1, 1), ME(3, double, 5, 1, 1), ME(3, double, 6, 1, 1), ME(3, double, 7, 1, 1),
+    ME(3, double, 8, 1, 1), ME(3, double, 9, 1, 1), ME(3, double, 10, 1, 1), ME(3, double, 11, 1, 1),
+    ME(3, double, 12, 1, 1), ME(3, double, 13, 1, 1), ME(3, double, 14, 1, 1), ME(3, double, 15, 1, 1),
+    ME(3, double, 16, 1, 1),
     // 2D
     ME(2, float, 1, 1, 1), ME(2, float, 1, 2, 1), ME(2, float, 1, 4, 1),
     ME(2, float, 2, 1, 1), ME(2, float, 2, 4, 1), ME(2, float, 3, 1, 1), ME(2, float, 3, 4, 1),
     ME(2, float, 4, 1, 1), ME(2, float, 4, 4, 1), ME(2, float, 5, 1, 1), ME(2, float, 6, 1, 1),
-    ME(2, float, 7, 1, 1), ME(2, float, 8, 1, 1),
+    ME(2, float, 7, 1, 1), ME(2, float, 8, 1, 1), ME(2, float, 9, 1, 1), ME(2, float, 10, 1, 1),
+    ME(2, float, 11, 1, 1), ME(2, float, 12, 1, 1), ME(2, float, 13, 1, 1), ME(2, float, 14, 1, 1),
+    ME(2, float, 15, 1, 1), ME(2, float, 16, 1, 1),
     ME(2, double, 1, 1, 1), ME(2, double, 1, 2, 1), ME(2, double, 2, 1, 1),
     ME(2, double, 2, 2, 1), ME(2, double, 4, 1, 1), ME(2, double, 4, 2, 1),
+    ME(2, double, 3, 1, 1), ME(2, double, 5, 1, 1), ME(2, double, 6, 1, 1), ME(2, double, 7, 1, 1),
+    ME(2, double, 8, 1, 1), ME(2, double, 9, 1, 1), ME(2, double, 10, 1, 1), ME(2, double, 11, 1, 1),
+    ME(2, double, 12, 1, 1), ME(2, double, 13, 1, 1), ME(2, double, 14, 1, 1), ME(2, double, 15, 1, 1),
+    ME(2, double, 16, 1, 1),
 };
 #undef ME
 
@@ -144,8 +156,7 @@ st_status resolve_march(Cfg &c)
     if (cx > 4) cx = 4;
     const MarchEntry *e = best_march(c.dims, f64, c.r, cx, ty);
     if (!e)
-        return fail(ST_EUNSUPPORTED, "SMEM_REGISTER is built for radius 1..8 (FP32) and 1,2,4 "
-                                     "(FP64); radius %d requested", c.r);
+        return fail(ST_EUNSUPPORTED, "SMEM_REGISTER is built for radius 1..16; radius %d requested", c.r);
     c.cx = e->cx;
     c.cy = e->ty;
     // Workgroup: block_x threads (a multiple of 64) along x, block_y waves of rows.

@@ -89,10 +89,10 @@ __device__ __forceinline__ T dpp_next(T v)  // lane i <- lane i+1 (rotating)
     }
 }
 
-// VGPRs a lane needs (fitted to the compiler's allocation: ring, partials,
-// prefetched rows, DPP copies of the neighbouring quads, temporaries), and
-// the workgroup size that leaves that many: 1024 threads leave 128, 768
-// leave 168, 512 leave 256.  Shapes above 250 are not built.
+// VGPRs a lane needs with the ring and partials shifted every plane (an
+// upper bound for the phase-unrolled loop; shapes above 250 are not built),
+// and the workgroup size whose register budget the kernel fits: 1024
+// threads leave 128 VGPRs, 768 leave 168, 512 leave 256.
 enum : int { RM_SWEEP = 0, RM_CORR = 1, RM_DEFECT = 2 };
 
 template <int R, int CX, typename T, int MODE = RM_SWEEP, int HS = 1>
@@ -105,12 +105,21 @@ constexpr int rows_vgprs()
 template <int R, int CX, typename T, int MODE = RM_SWEEP, int HS = 1>
 constexpr bool rows_supported() { return rows_vgprs<R, CX, T, MODE, HS>() <= 250; }
 
+// Workgroup size from the compiler's allocation of the phase-unrolled plane
+// loop (no spills at these bounds): every FP32 sweep and the 4-cell 16-bit
+// modes fit the 128 VGPRs of 16 waves (radius 4, 8 cells: 126); the 8-cell
+// 16-bit modes at radius 4 take 12 waves, FP64 12 waves with 4 cells
+// (radius 4: 148 VGPRs) or with 8 cells at radius 1, else 8.
 template <int R, int CX, typename T, int MODE = RM_SWEEP, int HS = 1>
 constexpr int rows_max_threads()
 {
-    return rows_vgprs<R, CX, T, MODE, HS>() <= 120   ? 1024
-           : rows_vgprs<R, CX, T, MODE, HS>() <= 160 ? 768
-                                                     : 512;
+    if constexpr (sizeof(T) == 8) {
+        return CX == 4 ? (R <= 3 ? 1024 : 768) : (R == 1 ? 768 : 512);
+    } else if constexpr (MODE == RM_SWEEP) {
+        return 1024;
+    } else {
+        return CX == 4 || R <= 3 ? 1024 : 768;
+    }
 }
 
 template <int OP, int R, int CX, typename T, int MODE = RM_SWEEP, bool F16 = false, int HS = 1>
@@ -127,7 +136,9 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
     QV *lds = reinterpret_cast<QV *>(lds_raw);
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wx = a.wx, wy = a.wy;
+    // 8 cells per lane are laid out only for rows of one wave, so the seam
+    // exchange of multi-wave rows compiles away there
+    const int wx = CX == 8 ? 1 : a.wx, wy = a.wy;
     const int wr = w / wx, wxp = w - wr * wx;  // row of the frame, part of the row
     const int QW = wx * H;                     // quads per LDS row
     const int lrows = wy + 2 * R;              // LDS rows per buffer: R halo, wy core, R halo
@@ -215,8 +226,31 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
 #pragma unroll
         for (int i = 0; i < HS; ++i)
             if (has_halo[i]) load_row(kin_lo, halo_off[i], hxo[i], ph[i]);
+        // correction sweep: the defect at the cells a step completes (plane
+        // m-R) is loaded one step ahead, with the plane prefetch
+        U4 dqn[H];
+        auto load_defect = [&](int k) {
+            if constexpr (MODE == RM_CORR) {
+                if (store_row && k >= k_out_lo && k < k_out_hi) {
+                    const uint16_t *p = aux + (long long)k * a.sz + own_off;
+#pragma unroll
+                    for (int h = 0; h < H; ++h) dqn[h] = *reinterpret_cast<const U4 *>(p + xo[h]);
+                }
+            }
+        };
+        load_defect(kin_lo - R);
 
-        for (int m = kin_lo; m < kin_hi; ++m) {
+        // planes in phases of R: the ring and the partial sums rotate
+        // through their R slots (logical entry i of phase rp in slot
+        // (rp + i) % R), so no values are shifted; the scheduling barrier
+        // keeps each plane's work in place, which also keeps the register
+        // need at the one-plane level
+        for (int m0 = kin_lo; m0 < kin_hi; m0 += R) {
+#pragma unroll
+        for (int rp = 0; rp < R; ++rp) {
+            const int m = m0 + rp;
+            if (m >= kin_hi) break;
+            __builtin_amdgcn_sched_barrier(0);
             T nw[CX];
 #pragma unroll
             for (int c = 0; c < CX; ++c) nw[c] = pf[c];
@@ -239,23 +273,19 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
                     buf[(hrow[i] * QW + hp[i] * H + h) * 64 + lane] = q;
                 }
             }
+            const int k = m - R;
+            const bool out_plane = store_row && k >= k_out_lo && k < k_out_hi;
+            U4 dq[H];
+            if constexpr (MODE == RM_CORR) {
+#pragma unroll
+                for (int h = 0; h < H; ++h) dq[h] = dqn[h];
+            }
             if (m + 1 < kin_hi) {
                 load_row(m + 1, own_off, xo, pf);
 #pragma unroll
                 for (int i = 0; i < HS; ++i)
                     if (has_halo[i]) load_row(m + 1, halo_off[i], hxo[i], ph[i]);
-            }
-            // correction sweep: the defect at the cell this step completes
-            // (plane m-R), loaded now and used after the barrier
-            const int k = m - R;
-            const bool out_plane = store_row && k >= k_out_lo && k < k_out_hi;
-            U4 dq[H];
-            if constexpr (MODE == RM_CORR) {
-                if (out_plane) {
-                    const uint16_t *p = aux + (long long)k * a.sz + own_off;
-#pragma unroll
-                    for (int h = 0; h < H; ++h) dq[h] = *reinterpret_cast<const U4 *>(p + xo[h]);
-                }
+                load_defect(k + 1);
             }
             __syncthreads();
 
@@ -340,23 +370,23 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
                     for (int j = 0; j < 4; ++j) s[4 * h + j] += q[j];
                 }
             }
-            // k-prev terms: ring[0] = plane m-R ... ring[R-1] = plane m-1
+            // k-prev terms: logical ring[0] = plane m-R ... ring[R-1] = plane m-1
 #pragma unroll
             for (int i = 0; i < R; ++i)
 #pragma unroll
-                for (int c = 0; c < CX; ++c) s[c] += ring[i][c];
+                for (int c = 0; c < CX; ++c) s[c] += ring[(rp + i) % R][c];
             // plane m is the next k-next term of P_{m-R} .. P_{m-1}
-            // (part[0] = P_{m-R}, which it completes)
+            // (logical part[0] = P_{m-R}, which it completes)
 #pragma unroll
             for (int i = 0; i < R; ++i)
 #pragma unroll
-                for (int c = 0; c < CX; ++c) part[i][c] += nw[c];
-            // P_{m-R} is complete: plane k = m-R, centre ring[0]
+                for (int c = 0; c < CX; ++c) part[(rp + i) % R][c] += nw[c];
+            // P_{m-R} is complete: plane k = m-R, centre logical ring[0]
             if (out_plane) {
                 if constexpr (MODE == RM_SWEEP) {
                     T res[CX];
 #pragma unroll
-                    for (int c = 0; c < CX; ++c) res[c] = finish<OP>(part[0][c], ring[0][c], kc);
+                    for (int c = 0; c < CX; ++c) res[c] = finish<OP>(part[rp][c], ring[rp][c], kc);
                     T *row = static_cast<T *>(out_) + (long long)k * a.sz + (long long)gy * a.nx;
 #pragma unroll
                     for (int h = 0; h < H; ++h) {
@@ -372,7 +402,7 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
                             QV q;
 #pragma unroll
                             for (int j = 0; j < 4; ++j)
-                                q[j] = interior1(xo[h] + j, a.nx, R) ? res[4 * h + j] : ring[0][4 * h + j];
+                                q[j] = interior1(xo[h] + j, a.nx, R) ? res[4 * h + j] : ring[rp][4 * h + j];
                             if (a.nt) __builtin_nontemporal_store(q, reinterpret_cast<QV *>(row + xo[h]));
                             else *reinterpret_cast<QV *>(row + xo[h]) = q;
                         } else if (act[h]) {
@@ -389,9 +419,9 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
                     for (int h = 0; h < H; ++h)
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
-                            const float sw = part[0][4 * h + j] * kc;
+                            const float sw = part[rp][4 * h + j] * kc;
                             if constexpr (MODE == RM_CORR) res[h][j] = rnd16<F16>(wid16<F16>(dq[h][j]) + sw);
-                            else res[h][j] = rnd16<F16>(sw - ring[0][4 * h + j]);
+                            else res[h][j] = rnd16<F16>(sw - ring[rp][4 * h + j]);
                         }
                     uint16_t *row = static_cast<uint16_t *>(out_) + (long long)k * a.sz + (long long)gy * a.nx;
 #pragma unroll
@@ -411,19 +441,13 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
                     }
                 }
             }
-            // shift: ring <- planes m-R+1..m, partials <- P_{m-R+1}..P_m
-#pragma unroll
-            for (int i = 0; i + 1 < R; ++i)
-#pragma unroll
-                for (int c = 0; c < CX; ++c) {
-                    ring[i][c] = ring[i + 1][c];
-                    part[i][c] = part[i + 1][c];
-                }
+            // plane m and P_m take the slot of plane m-R and P_{m-R}
 #pragma unroll
             for (int c = 0; c < CX; ++c) {
-                ring[R - 1][c] = nw[c];
-                part[R - 1][c] = s[c];
+                ring[rp][c] = nw[c];
+                part[rp][c] = s[c];
             }
+        }
         }
         __syncthreads();  // the next tile rewrites the LDS planes
     }
@@ -556,7 +580,10 @@ static st_status rows_resolve_mode(Cfg &c, int mode, bool f16)
         wy = c.ny < rows_min ? rows_min : c.ny;
     }
     const int es = (c.dtype == ST_F64 && mode == RM_SWEEP) ? 8 : 4;
-    c.lds = 2 * (wy + 2 * c.r) * wx * (cx / 4) * 64 * 4 * es;
+    auto lds_of = [&](int rows) { return 2 * (rows + 2 * c.r) * wx * (cx / 4) * 64 * 4 * es; };
+    // the heuristic frame shrinks until its double-buffered planes fit
+    while ((c.heuristic || c.by <= 0) && lds_of(wy) > kMaxLds && wy > rows_min) --wy;
+    c.lds = lds_of(wy);
     if (c.lds > kMaxLds) return fail(ST_ELDS, "row-streaming form needs %d bytes of LDS", c.lds);
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -639,6 +666,21 @@ static st_status rows_launch(const Cfg &c, const void *fn, const void *in, void 
     void *args[] = {&in, &out, &aux, &a, karg};
     if (sweeps > 1) {
         e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(nthreads), args, c.lds, s);
+        if (e == hipErrorIllegalState) {
+            // the device takes no cooperative launch from this process now:
+            // the same sweeps, one launch each, the buffers swapped between
+            (void)hipGetLastError();
+            const void *a_in = in;
+            void *a_out = out;
+            for (int i = 0; i < sweeps; ++i) {
+                st_status st = rows_launch(c, fn, a_in, a_out, aux, s, 1);
+                if (st != ST_OK) return st;
+                void *t = const_cast<void *>(a_in);
+                a_in = a_out;
+                a_out = t;
+            }
+            return ST_OK;
+        }
         if (e != hipSuccess) return hip_check(e, "hipLaunchCooperativeKernel(rows)");
     } else {
         e = hipLaunchKernel(fn, dim3(grid), dim3(nthreads), args, c.lds, s);

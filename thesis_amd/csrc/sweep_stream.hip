@@ -11,8 +11,16 @@
 //     so two sweeps fuse into one pass with no boundary reloads.  Passes of
 //     two sweeps run first, then (iters mod 4) single-sweep passes, which
 //     keeps the pass count's parity equal to the iteration count's;
-//   * different shells (ST_INIT_REFERENCE leaves u2 zero): one sweep per
-//     pass, reading the whole of buffer s%2, as the reference does.
+//   * different shells (ST_INIT_REFERENCE leaves u2 zero): the passes fuse
+//     two sweeps as well.  Level 0 of a fused pass is the state after an
+//     even number of sweeps, whose shell is u1's; level 1 has u2's shell.
+//     The pass reads the interior from the buffer X it streams, but takes
+//     every level-0 shell value from u1 and every level-1 shell value from
+//     u2: z-shell planes and y-shell rows come from those buffers directly,
+//     the x-shell cells (columns 0 and nx-1 of each row) from a compact
+//     table of both buffers' x-shell values that k_shell_eq writes, so the
+//     pass moves the same HBM bytes as with equal shells.  Only interior
+//     cells are stored, so both shells stay untouched.
 //
 // Layout: a workgroup of WY waves owns a frame of 2*WY full rows (a wave
 // holds two rows, each lane CX cells of both, in quads of four consecutive
@@ -32,6 +40,8 @@
 // their core rows are stored; z chunks overlap by two planes likewise.
 #include <hip/hip_cooperative_groups.h>
 
+#include <algorithm>
+
 #include "st_device.hpp"
 #include "st_internal.hpp"
 
@@ -45,18 +55,25 @@ struct SmArgs {
     int core_h;           // stored rows per frame (frame rows - 4)
     int tiles_y, zchunk, ntiles;
     int iters;
-    int n2, n1;           // equal shells: passes of two sweeps, then of one
+    int n2, n1;           // passes of two sweeps, then of one
     int p0, p1;           // passes of this launch: [p0, min(p1, passes))
-    int fuse_diff;        // fuse two sweeps with different shells too (the
-                          // host runs k_stream_fix after each such pass)
     const int *shell_eq;  // kShellBlocks flags from k_shell_eq
+    const float *xs;      // x-shell table of both buffers (xshell_index)
 };
 
+// Index of the x-shell value of buffer b (0: u1, 1: u2) at row j of plane
+// k, column 0 (e = 0) or nx-1 (e = 1), in the table k_shell_eq writes.
+__device__ __forceinline__ long long xshell_index(int b, int k, int j, int e, int ny, int nz)
+{
+    return (((long long)b * nz + k) * ny + j) * 2 + e;
+}
+
 // part[b] = 1 when every shell cell block b visits holds the same bits in
-// u1 and u2.
+// u1 and u2; the x-shell cells of both buffers (columns 0 and nx-1 of every
+// row) are copied into the table xs on the way.
 __global__ __launch_bounds__(256) void k_shell_eq(const float *__restrict__ u1,
                                                   const float *__restrict__ u2, int nx, int ny,
-                                                  int nz, int *__restrict__ part)
+                                                  int nz, int *__restrict__ part, float *__restrict__ xs)
 {
     const long long A = (long long)nx * ny;
     const long long P = 2LL * nx + 2LL * (ny - 2);  // shell cells of an inner plane
@@ -64,14 +81,17 @@ __global__ __launch_bounds__(256) void k_shell_eq(const float *__restrict__ u1,
     int ok = 1;
     for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) {
         long long idx;
+        int x, y, z;
         if (i < 2 * A) {
             const long long j = i < A ? i : i - A;
-            idx = (i < A ? 0 : (long long)(nz - 1) * A) + j;
+            z = i < A ? 0 : nz - 1;
+            y = (int)(j / nx);
+            x = (int)(j - (long long)y * nx);
+            idx = (long long)z * A + j;
         } else {
             const long long q = i - 2 * A;
-            const long long z = 1 + q / P;
+            z = 1 + (int)(q / P);
             long long j = q % P;
-            int x, y;
             if (j < 2LL * nx) {
                 y = j < nx ? 0 : ny - 1;
                 x = (int)(j < nx ? j : j - nx);
@@ -80,9 +100,15 @@ __global__ __launch_bounds__(256) void k_shell_eq(const float *__restrict__ u1,
                 y = 1 + (int)(j >> 1);
                 x = (j & 1) ? nx - 1 : 0;
             }
-            idx = z * A + (long long)y * nx + x;
+            idx = (long long)z * A + (long long)y * nx + x;
         }
-        ok &= __float_as_uint(u1[idx]) == __float_as_uint(u2[idx]);
+        const float a = u1[idx], b = u2[idx];
+        ok &= __float_as_uint(a) == __float_as_uint(b);
+        if (x == 0 || x == nx - 1) {
+            const int e = x == 0 ? 0 : 1;
+            xs[xshell_index(0, z, y, e, ny, nz)] = a;
+            xs[xshell_index(1, z, y, e, ny, nz)] = b;
+        }
     }
     ok = __syncthreads_and(ok);
     if (threadIdx.x == 0) part[blockIdx.x] = ok;
@@ -123,28 +149,99 @@ struct SmLane {
 template <int RPW>
 struct SmTile {
     int rowoff[RPW];            // clamped row offsets within a plane
+    int rowc[RPW];              // clamped row index
     bool yshell[RPW];           // row outside the interior (wave-uniform)
+    bool ytrue[RPW];            // row 0 or ny-1 of the grid (wave-uniform)
     bool ystore[RPW];           // row is an interior core row (wave-uniform)
     int z0, z1;
 };
 
+// Shell values of a fused pass with different shells (unused with equal
+// shells).  SmShell0 travels with a prefetched level-0 plane: its x-shell
+// values from u1, which replace the streamed buffer's when the plane is
+// consumed.  SmShell1 holds the level-1 shell values of one plane from u2:
+// the x-shell values of each row and the grid's y-shell rows; it is loaded
+// one step before the step that finishes that plane.
+template <int RPW>
+struct SmShell0 {
+    float2 x0[RPW];     // u1 at columns 0 and nx-1
+};
 template <int CX, int RPW>
-__device__ __forceinline__ void sm_load(SmPlane<CX, RPW> &d, const float *__restrict__ X, const SmArgs &a,
-                                        const SmTile<RPW> &t, const SmLane<CX> &l, int k)
+struct SmShell1 {
+    float2 x1[RPW];     // u2 at columns 0 and nx-1
+    float y1[RPW][CX];  // u2, rows with ytrue
+};
+
+// Level-0 plane k (clamped into the grid) from X.  DIFF: shell planes and
+// shell rows from S0 (u1) instead, and the x-shell values of each row from
+// the table; they are patched in when the plane is consumed (sm_take), so
+// that the loads stay in flight meanwhile.
+template <int CX, int RPW, bool DIFF>
+__device__ __forceinline__ void sm_load(SmPlane<CX, RPW> &d, SmShell0<RPW> &sh, const float *__restrict__ X,
+                                        const float *__restrict__ S0, const float *__restrict__ xs,
+                                        const SmArgs &a, const SmTile<RPW> &t, const SmLane<CX> &l, int k)
 {
     typedef float V4 __attribute__((ext_vector_type(4)));
     k = min(max(k, 0), a.nz - 1);
-    const float *base = X + (long long)k * a.sz;
+    const long long po = (long long)k * a.sz;
+    const float *base = (DIFF && (k == 0 || k == a.nz - 1) ? S0 : X) + po;
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
+        const float *rb = (DIFF && t.yshell[r] ? S0 + po : base) + t.rowoff[r];
 #pragma unroll
         for (int h = 0; h < CX / 4; ++h) {
-            const V4 q = *reinterpret_cast<const V4 *>(base + t.rowoff[r] + l.xo[h]);
+            const V4 q = *reinterpret_cast<const V4 *>(rb + l.xo[h]);
             d.v[r][4 * h + 0] = q[0];
             d.v[r][4 * h + 1] = q[1];
             d.v[r][4 * h + 2] = q[2];
             d.v[r][4 * h + 3] = q[3];
         }
+    }
+    if constexpr (DIFF) {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+            sh.x0[r] = *reinterpret_cast<const float2 *>(xs + xshell_index(0, k, t.rowc[r], 0, a.ny, a.nz));
+    }
+}
+
+// Level-1 shell values of plane k from u2 (S1): x-shell values from the
+// table, the y-shell rows from S1.
+template <int CX, int RPW>
+__device__ __forceinline__ void sm_load_sh1(SmShell1<CX, RPW> &sh, const float *__restrict__ S1,
+                                            const float *__restrict__ xs, const SmArgs &a, const SmTile<RPW> &t,
+                                            const SmLane<CX> &l, int k)
+{
+    typedef float V4 __attribute__((ext_vector_type(4)));
+    k = min(max(k, 0), a.nz - 1);
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        sh.x1[r] = *reinterpret_cast<const float2 *>(xs + xshell_index(1, k, t.rowc[r], 0, a.ny, a.nz));
+        if (t.ytrue[r]) {
+            const float *rb = S1 + (long long)k * a.sz + t.rowoff[r];
+#pragma unroll
+            for (int h = 0; h < CX / 4; ++h) {
+                const V4 q = *reinterpret_cast<const V4 *>(rb + l.xo[h]);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) sh.y1[r][4 * h + j] = q[j];
+            }
+        }
+    }
+}
+
+// The loaded plane with its level-0 x-shell cells replaced by u1's (DIFF).
+template <int CX, int RPW, bool DIFF>
+__device__ __forceinline__ void sm_take(SmPlane<CX, RPW> &d, const SmPlane<CX, RPW> &p,
+                                        const SmShell0<RPW> &sh, const SmLane<CX> &l)
+{
+    d = p;
+    if constexpr (DIFF) {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int h = 0; h < CX / 4; ++h) {
+                d.v[r][4 * h] = l.lo[h] ? sh.x0[r].x : d.v[r][4 * h];
+                d.v[r][4 * h + 3] = l.hi[h] ? sh.x0[r].y : d.v[r][4 * h + 3];
+            }
     }
 }
 
@@ -231,31 +328,49 @@ __device__ __forceinline__ void sm_partial(SmPlane<CX, RPW> &out, const SmPlane<
 }
 
 // out = finish(partial + z-next) with cells outside the interior taking the
-// centre value (equal shells: a shell cell never changes).
-template <int OP, int CX, int RPW>
+// centre value (equal shells: a shell cell never changes).  SH1 (level 1 of
+// a fused pass with different shells): shell cells take u2's values instead
+// (the shell plane k1 is loaded from S1 here; the x-shell and y-shell row
+// values travel with the centre plane in `sh`).
+template <int OP, int CX, int RPW, bool SH1>
 __device__ __forceinline__ void sm_finish(SmPlane<CX, RPW> &out, const SmPlane<CX, RPW> &part,
                                           const SmPlane<CX, RPW> &zp, const SmPlane<CX, RPW> &cen,
                                           float kc, bool zshell, const SmTile<RPW> &t,
-                                          const SmLane<CX> &l)
+                                          const SmLane<CX> &l, const SmShell1<CX, RPW> &sh,
+                                          const float *__restrict__ S1, const SmArgs &a, int k)
 {
+    typedef float V4 __attribute__((ext_vector_type(4)));
 #pragma unroll
     for (int r = 0; r < RPW; ++r)
 #pragma unroll
         for (int c = 0; c < CX; ++c) out.v[r][c] = finish<OP>(part.v[r][c] + zp.v[r][c], cen.v[r][c], kc);
     if (zshell) {
-        out = cen;
+        if (SH1 && (k == 0 || k == a.nz - 1)) {
+            // a z-shell plane: u2's values (twice per z column of tiles)
+            const float *base = S1 + (long long)k * a.sz;
+#pragma unroll
+            for (int r = 0; r < RPW; ++r)
+#pragma unroll
+                for (int h = 0; h < CX / 4; ++h) {
+                    const V4 q = *reinterpret_cast<const V4 *>(base + t.rowoff[r] + l.xo[h]);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) out.v[r][4 * h + j] = q[j];
+                }
+        } else {
+            out = cen;
+        }
         return;
     }
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
         if (t.yshell[r]) {
 #pragma unroll
-            for (int c = 0; c < CX; ++c) out.v[r][c] = cen.v[r][c];
+            for (int c = 0; c < CX; ++c) out.v[r][c] = (SH1 && t.ytrue[r]) ? sh.y1[r][c] : cen.v[r][c];
         } else {
 #pragma unroll
             for (int h = 0; h < CX / 4; ++h) {
-                const float a0 = cen.v[r][4 * h], b0 = out.v[r][4 * h];
-                const float a1 = cen.v[r][4 * h + 3], b1 = out.v[r][4 * h + 3];
+                const float a0 = SH1 ? sh.x1[r].x : cen.v[r][4 * h], b0 = out.v[r][4 * h];
+                const float a1 = SH1 ? sh.x1[r].y : cen.v[r][4 * h + 3], b1 = out.v[r][4 * h + 3];
                 out.v[r][4 * h] = l.lo[h] ? a0 : b0;
                 out.v[r][4 * h + 3] = l.hi[h] ? a1 : b1;
             }
@@ -294,11 +409,15 @@ __device__ __forceinline__ void sm_store(float *__restrict__ Y, const SmPlane<CX
 }
 
 // One pass of T (1 or 2) sweeps X -> Y over all tiles of this workgroup.
-template <int OP, int CX, int RPW, int T>
+// DIFF (T = 2, the buffers' shells differ): level-0 shell values from S0
+// (u1), level-1 shell values from S1 (u2).
+template <int OP, int CX, int RPW, int T, bool DIFF>
 __device__ __forceinline__ void sm_pass(const float *__restrict__ X, float *__restrict__ Y,
-                                        const SmArgs &a, float kc, float4 *lds, unsigned bid,
-                                        unsigned nblk, bool same)
+                                        const float *__restrict__ S0, const float *__restrict__ S1,
+                                        const float *__restrict__ xs, const SmArgs &a, float kc, float4 *lds,
+                                        unsigned bid, unsigned nblk, bool same)
 {
+    static_assert(!DIFF || T == 2, "single-sweep passes read the shell of the buffer they stream");
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wy = blockDim.x >> 6;
@@ -321,11 +440,14 @@ __device__ __forceinline__ void sm_pass(const float *__restrict__ X, float *__re
         const int fy0 = tyi * a.core_h - 2;
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
+            // wave-uniform row constants, kept in scalar registers
             const int fr = w * RPW + r;  // frame row
-            const int gy = fy0 + fr;
+            const int gy = __builtin_amdgcn_readfirstlane(fy0 + fr);
             const int gyc = min(max(gy, 0), a.ny - 1);
             t.rowoff[r] = gyc * a.nx;
+            t.rowc[r] = gyc;
             t.yshell[r] = gy <= 0 || gy >= a.ny - 1;
+            t.ytrue[r] = gy == 0 || gy == a.ny - 1;
             t.ystore[r] = !t.yshell[r] && fr >= 2 && fr < 2 + a.core_h;
         }
         t.z0 = tzi * a.zchunk;
@@ -334,20 +456,29 @@ __device__ __forceinline__ void sm_pass(const float *__restrict__ X, float *__re
         // Pipeline state: the prefetched level-0 plane, the previous level-0
         // and level-1 planes, and the running partial sums P1/P2 (updated in
         // place once the finished plane has taken their old values).
+        // (DIFF: the level-0 x-shell values of the prefetched plane ride
+        // along in pfs; sh1 holds the level-1 shell values of the plane the
+        // step finishes at level 1)
         SmPlane<CX, RPW> pf, c0, c1, P1, P2;
+        SmShell0<RPW> pfs;
+        SmShell1<CX, RPW> sh1;
 #pragma unroll
         for (int r = 0; r < RPW; ++r)
 #pragma unroll
             for (int c = 0; c < CX; ++c) c0.v[r][c] = c1.v[r][c] = P1.v[r][c] = P2.v[r][c] = 0.0f;
         const int kin_lo = t.z0 - T, kin_hi = t.z1 + T;
-        sm_load<CX, RPW>(pf, X, a, t, l, kin_lo);
+        sm_load<CX, RPW, DIFF>(pf, pfs, X, S0, xs, a, t, l, kin_lo);
+        if constexpr (DIFF) sm_load_sh1<CX, RPW>(sh1, S1, xs, a, t, l, kin_lo - 1);
         for (int kin = kin_lo; kin < kin_hi; ++kin) {
-            const SmPlane<CX, RPW> nw0 = pf;
-            if (kin + 1 < kin_hi) sm_load<CX, RPW>(pf, X, a, t, l, kin + 1);
+            SmPlane<CX, RPW> nw0;
+            sm_take<CX, RPW, DIFF>(nw0, pf, pfs, l);
+            if (kin + 1 < kin_hi) sm_load<CX, RPW, DIFF>(pf, pfs, X, S0, xs, a, t, l, kin + 1);
             // level 1, plane kin-1
             const int k1 = kin - 1;
             SmPlane<CX, RPW> o1;
-            sm_finish<OP, CX, RPW>(o1, P1, nw0, c0, kc, k1 <= 0 || k1 >= a.nz - 1, t, l);
+            sm_finish<OP, CX, RPW, DIFF>(o1, P1, nw0, c0, kc, k1 <= 0 || k1 >= a.nz - 1, t, l, sh1, S1, a, k1);
+            // the level-1 shell values the next step needs (plane kin)
+            if constexpr (DIFF) sm_load_sh1<CX, RPW>(sh1, S1, xs, a, t, l, kin);
             // one sweep: alternate the two regions so that a plane's LDS
             // writes never meet the previous plane's reads
             sm_partial<CX, RPW>(P1, nw0, c0, (T == 1 && (kin & 1)) ? reg1 : reg0, w, wy, lane);
@@ -358,7 +489,8 @@ __device__ __forceinline__ void sm_pass(const float *__restrict__ X, float *__re
                 // level 2, plane kin-2
                 const int k2 = kin - 2;
                 SmPlane<CX, RPW> o2;
-                sm_finish<OP, CX, RPW>(o2, P2, o1, c1, kc, k2 <= 0 || k2 >= a.nz - 1, t, l);
+                sm_finish<OP, CX, RPW, false>(o2, P2, o1, c1, kc, k2 <= 0 || k2 >= a.nz - 1, t, l, sh1, S1, a,
+                                              k2);
                 sm_partial<CX, RPW>(P2, o1, c1, reg1, w, wy, lane);
                 c1 = o1;
                 sm_store<CX, RPW>(Y, o2, a, t, l, k2);
@@ -370,157 +502,43 @@ __device__ __forceinline__ void sm_pass(const float *__restrict__ X, float *__re
 
 // Threads per workgroup from the state per lane (five planes of CX*RPW
 // values plus the plane in flight): 8 values fit the 128 VGPRs of 16 waves,
-// 16 the 168 of 12 waves, 32 the 256 of 8 waves.
-template <int CX, int RPW>
-constexpr int sm_max_threads() { return CX * RPW <= 8 ? 1024 : CX * RPW <= 16 ? 768 : 512; }
+// 16 the 168 of 12 waves, 32 the 256 of 8 waves.  The kernel for different
+// shells (DIFF) also carries the shell values that travel with two of the
+// planes: 8 values fit 12 waves, 16 and more 8 waves.
+template <int CX, int RPW, bool DIFF>
+constexpr int sm_max_threads()
+{
+    return DIFF ? (CX * RPW <= 8 ? 768 : 512) : (CX * RPW <= 8 ? 1024 : CX * RPW <= 16 ? 768 : 512);
+}
 
-template <int OP, int CX, int RPW>
-__global__ __launch_bounds__((sm_max_threads<CX, RPW>())) void k_stream(float *u1, float *u2, SmArgs a,
-                                                                      float kc)
+// Two kernels share every launch of the streaming form: the one for equal
+// shells (DIFF false) and the one for different shells (DIFF true); both
+// read the comparison flags and the one whose case does not hold returns at
+// once.  Single-sweep passes read the shell of the buffer they stream, which
+// is right for either case, and run in whichever kernel stays.
+template <int OP, int CX, int RPW, bool DIFF>
+__global__ __launch_bounds__((sm_max_threads<CX, RPW, DIFF>())) void k_stream(float *__restrict__ u1,
+                                                                            float *__restrict__ u2, SmArgs a,
+                                                                            float kc)
 {
     extern __shared__ float4 lds[];
     int ok = 1;
     for (int i = threadIdx.x; i < kShellBlocks; i += blockDim.x) ok &= a.shell_eq[i];
     const bool same = __syncthreads_and(ok) != 0;
+    if (same == DIFF) return;
     const unsigned nblk = gridDim.x;
     const unsigned bid = xcd_remap(blockIdx.x, nblk);
-    const bool fuse = same || a.fuse_diff;
-    const int npass = min(fuse ? a.n2 + a.n1 : a.iters, a.p1);
+    const int npass = min(a.n2 + a.n1, a.p1);
     for (int p = a.p0; p < npass; ++p) {
         const float *X = (p & 1) ? u2 : u1;
         float *Y = (p & 1) ? u1 : u2;
-        if (fuse && p < a.n2) sm_pass<OP, CX, RPW, 2>(X, Y, a, kc, lds, bid, nblk, same);
-        else sm_pass<OP, CX, RPW, 1>(X, Y, a, kc, lds, bid, nblk, same);
+        if (p < a.n2) sm_pass<OP, CX, RPW, 2, DIFF>(X, Y, u1, u2, a.xs, a, kc, lds, bid, nblk, same);
+        else sm_pass<OP, CX, RPW, 1, false>(X, Y, u1, u2, a.xs, a, kc, lds, bid, nblk, same);
         if (p + 1 < npass) cooperative_groups::this_grid().sync();
     }
 }
 
-// Fused passes with different shells.  Level 0 of a fused pass is the state
-// after an even number of sweeps, whose shell is u1's; level 1 has u2's
-// shell.  The fused pass takes every shell value from the buffer it reads,
-// so its level-2 result is exact except within two cells of the shell
-// (level 1 is wrong next to the shell when the pass reads u2, and level 1's
-// own shell is wrong when it reads u1).  k_stream_fix recomputes those
-// cells, the interior cells with an index 1, 2, n-3 or n-2 on some axis,
-// from level 0 exactly as the per-sweep ping-pong does: level-0 shell from
-// u1, interior from X; level-1 shell from u2, interior by the same sums in
-// the same order as every sweep kernel.  Both shells stay untouched (the
-// fused pass stores interior cells only when the shells differ).
-struct FixArgs {
-    int nx, ny, nz;
-    int kb[4], jb[4], ib[4];  // band indices per axis (1, 2, n-3, n-2, deduplicated)
-    int nkb, njb, nib;
-    long long plane_a;        // cells of a band plane: (nx-2)(ny-2)
-    long long plane_b;        // band cells of an inner plane
-    long long total;
-};
-
-template <int OP>
-__global__ __launch_bounds__(256) void k_stream_fix(const float *__restrict__ X, const float *__restrict__ S0,
-                                                    const float *__restrict__ S1, float *__restrict__ Y,
-                                                    FixArgs f, const int *__restrict__ shell_eq, float kc)
-{
-    int ok = 1;
-    for (int i = threadIdx.x; i < kShellBlocks; i += blockDim.x) ok &= shell_eq[i];
-    if (__syncthreads_and(ok)) return;  // equal shells: the fused pass was exact
-    const int nx = f.nx, ny = f.ny, nz = f.nz;
-    const long long sy = nx, sz = (long long)nx * ny, last = sz * nz - 1;
-    const int wx = nx - 2;
-    for (long long t = blockIdx.x * 256LL + threadIdx.x; t < f.total; t += 256LL * gridDim.x) {
-        int i, j, k;
-        if (t < f.nkb * f.plane_a) {
-            // planes k in the band: every interior cell
-            const int q = (int)(t / f.plane_a);
-            const long long r = t - q * f.plane_a;
-            k = f.kb[q];
-            j = 1 + (int)(r / wx);
-            i = 1 + (int)(r % wx);
-        } else {
-            // inner planes: band rows whole, inner rows at their band columns
-            const long long t2 = t - f.nkb * f.plane_a;
-            const long long q = t2 / f.plane_b;
-            long long r = t2 - q * f.plane_b;
-            k = 3 + (int)q;
-            if (r < (long long)f.njb * wx) {
-                j = f.jb[r / wx];
-                i = 1 + (int)(r % wx);
-            } else {
-                r -= (long long)f.njb * wx;
-                j = 3 + (int)(r / f.nib);
-                i = f.ib[r % f.nib];
-            }
-        }
-        // shell flags of the offsets -2..2 on each axis (the 25 level-0
-        // cells within distance 2 of the cell; offsets reaching outside the
-        // grid are never read: they belong to shell neighbours, whose level-1
-        // value is the stored shell)
-        bool xs[5], ys[5], zs[5];
-#pragma unroll
-        for (int d = 0; d < 5; ++d) {
-            xs[d] = i + d - 2 == 0 || i + d - 2 == nx - 1;
-            ys[d] = j + d - 2 == 0 || j + d - 2 == ny - 1;
-            zs[d] = k + d - 2 == 0 || k + d - 2 == nz - 1;
-        }
-        const long long base = i + (long long)j * sy + k * sz;
-        auto lev0 = [&](int dx, int dy, int dz) -> float {
-            const bool sh = xs[dx + 2] || ys[dy + 2] || zs[dz + 2];
-            // clamped into the field even where a compiler might read ahead
-            // of the shell test in lev1
-            const long long id = min(max(base + dx + dy * sy + dz * sz, 0LL), last);
-            return (sh ? S0 : X)[id];
-        };
-        auto lev1 = [&](int dx, int dy, int dz) -> float {
-            if (xs[dx + 2] || ys[dy + 2] || zs[dz + 2]) return S1[base + dx + dy * sy + dz * sz];
-            float u = 0.0f;
-            u += lev0(dx - 1, dy, dz);
-            u += lev0(dx + 1, dy, dz);
-            u += lev0(dx, dy - 1, dz);
-            u += lev0(dx, dy + 1, dz);
-            u += lev0(dx, dy, dz - 1);
-            u += lev0(dx, dy, dz + 1);
-            return finish<OP>(u, lev0(dx, dy, dz), kc);
-        };
-        float u = 0.0f;
-        u += lev1(-1, 0, 0);
-        u += lev1(1, 0, 0);
-        u += lev1(0, -1, 0);
-        u += lev1(0, 1, 0);
-        u += lev1(0, 0, -1);
-        u += lev1(0, 0, 1);
-        Y[base] = finish<OP>(u, lev1(0, 0, 0), kc);
-    }
-}
-
 // ---------------------------------------------------------------- host side
-
-// Band indices of one axis of length n (interior 1..n-2): 1, 2, n-3, n-2,
-// deduplicated; the inner indices 3..n-4 are the rest.
-static int band_axis(int n, int *b)
-{
-    int cnt = 0;
-    const int cand[4] = {1, 2, n - 3, n - 2};
-    for (int v : cand) {
-        if (v < 1 || v > n - 2) continue;
-        bool dup = false;
-        for (int q = 0; q < cnt; ++q) dup |= b[q] == v;
-        if (!dup) b[cnt++] = v;
-    }
-    return cnt;
-}
-
-static FixArgs fix_args(const Cfg &c)
-{
-    FixArgs f{};
-    f.nx = c.nx; f.ny = c.ny; f.nz = c.nz;
-    f.nkb = band_axis(c.nz, f.kb);
-    f.njb = band_axis(c.ny, f.jb);
-    f.nib = band_axis(c.nx, f.ib);
-    const long long inner_k = c.nz > 6 ? c.nz - 6 : 0, inner_j = c.ny > 6 ? c.ny - 6 : 0;
-    f.plane_a = (long long)(c.nx - 2) * (c.ny - 2);
-    f.plane_b = (long long)f.njb * (c.nx - 2) + inner_j * f.nib;
-    f.total = f.nkb * f.plane_a + inner_k * f.plane_b;
-    return f;
-}
 
 static int stream_cx(const Cfg &c)
 {
@@ -534,25 +552,28 @@ bool stream_eligible(const Cfg &c)
            c.nx >= 8 && c.ny >= 3 && c.nz >= 3 && stream_cx(c) != 0;
 }
 
-template <int CX, int RPW>
+template <int CX, int RPW, bool DIFF>
 static const void *stream_fn_t(int op)
 {
-    return op == OP_THESIS ? reinterpret_cast<const void *>(&k_stream<OP_THESIS, CX, RPW>)
-                           : reinterpret_cast<const void *>(&k_stream<OP_JACOBI, CX, RPW>);
+    return op == OP_THESIS ? reinterpret_cast<const void *>(&k_stream<OP_THESIS, CX, RPW, DIFF>)
+                           : reinterpret_cast<const void *>(&k_stream<OP_JACOBI, CX, RPW, DIFF>);
 }
 
 // Instantiated shapes: 8 cells x 2 rows (nx up to 512), 4 cells x 2 or 4
 // rows (nx up to 256).  8 x 4 needs more than 256 VGPRs and is not built.
-static const void *stream_fn(int op, int cx, int rpw)
+static const void *stream_fn(int op, int cx, int rpw, bool diff)
 {
-    if (cx == 8) return stream_fn_t<8, 2>(op);
-    return rpw == 4 ? stream_fn_t<4, 4>(op) : stream_fn_t<4, 2>(op);
+    if (cx == 8) return diff ? stream_fn_t<8, 2, true>(op) : stream_fn_t<8, 2, false>(op);
+    if (rpw == 4) return diff ? stream_fn_t<4, 4, true>(op) : stream_fn_t<4, 4, false>(op);
+    return diff ? stream_fn_t<4, 2, true>(op) : stream_fn_t<4, 2, false>(op);
 }
 
+// Waves per frame that both kernels of a shape take.
 static int stream_max_waves(int cx, int rpw)
 {
-    if (cx == 8) return sm_max_threads<8, 2>() / 64;
-    return (rpw == 4 ? sm_max_threads<4, 4>() : sm_max_threads<4, 2>()) / 64;
+    if (cx == 8) return std::min(sm_max_threads<8, 2, false>(), sm_max_threads<8, 2, true>()) / 64;
+    if (rpw == 4) return std::min(sm_max_threads<4, 4, false>(), sm_max_threads<4, 4, true>()) / 64;
+    return std::min(sm_max_threads<4, 2, false>(), sm_max_threads<4, 2, true>()) / 64;
 }
 
 // Tiles of one pass for `wy` waves of `rpw` rows per frame: y frames times
@@ -616,8 +637,12 @@ st_status resolve_stream(Cfg &c)
         // tiles fill the CUs best
         const int wfit = (c.ny + 4 + rpw - 1) / rpw;
         best = stream_plan(c, wfit < wmax ? (wfit < wmin ? wmin : wfit) : wmax, rpw, cus, chunk_req);
+        // candidate frame heights: down to two thirds of the largest, and
+        // not below 8 waves where 8 fit (fewer waves lose the latency
+        // hiding; 512^3: 6 waves x 3 chunks ran 21% slower than 8 x 5)
+        const int wlo = std::max(std::max(wmin, (wmax * 2) / 3), std::min(wmax, 8));
         if (wfit > wmax)
-            for (int wy = wmax - 1; wy >= (wmax * 2) / 3 && wy >= wmin; --wy) {
+            for (int wy = wmax - 1; wy >= wlo; --wy) {
                 const SmPlan q = stream_plan(c, wy, rpw, cus, chunk_req);
                 if (q.cost < best.cost * 0.97) best = q;  // fewer waves must pay for themselves
             }
@@ -633,7 +658,7 @@ st_status resolve_stream(Cfg &c)
         if (chunk_req <= 0 && 2LL * c.nx * c.ny * c.nz * 4 >= (640LL << 20)) {
             SmPlan pick{};
             double pick_amp = 0;
-            for (int wy = wmax; wy >= (wmax * 2) / 3 && wy >= wmin; --wy) {
+            for (int wy = wmax; wy >= wlo; --wy) {
                 const int core_h = wy * rpw - 4;
                 const int tiles_y = (c.ny + core_h - 1) / core_h;
                 for (int nch = 1; nch * tiles_y <= cus; ++nch) {
@@ -670,15 +695,21 @@ st_status run_stream(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2
 {
     *in_u2 = c.iters & 1;
     if (c.iters <= 0) return ST_OK;
+    // workspace: the shell-comparison flags, then the x-shell table of both
+    // buffers (2 x 2 values per row)
+    const size_t flag_bytes = 1024;
+    static_assert(kShellBlocks * sizeof(int) <= flag_bytes, "flags fit their part");
+    const size_t xs_bytes = (size_t)2 * 2 * c.ny * c.nz * sizeof(float);
     void *ws = nullptr;
-    st_status st = workspace_slot(3, kShellBlocks * sizeof(int), s, &ws);
+    st_status st = workspace_slot(3, flag_bytes + xs_bytes, s, &ws);
     if (st != ST_OK) return st;
+    float *xs = reinterpret_cast<float *>(static_cast<unsigned char *>(ws) + flag_bytes);
     hipLaunchKernelGGL(k_shell_eq, dim3(kShellBlocks), dim3(256), 0, s, static_cast<const float *>(u1),
-                       static_cast<const float *>(u2), c.nx, c.ny, c.nz, static_cast<int *>(ws));
+                       static_cast<const float *>(u2), c.nx, c.ny, c.nz, static_cast<int *>(ws), xs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_check(e, "shell comparison launch");
 
-    const void *fn = stream_fn(c.op, c.cx, c.cy);
+    const void *fns[2] = {stream_fn(c.op, c.cx, c.cy, false), stream_fn(c.op, c.cx, c.cy, true)};
     SmArgs a;
     a.nx = c.nx; a.ny = c.ny; a.nz = c.nz;
     a.sz = (long long)c.nx * c.ny;
@@ -690,53 +721,67 @@ st_status run_stream(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2
     a.n1 = c.iters % 4;
     a.n2 = (c.iters - a.n1) / 2;
     a.shell_eq = static_cast<const int *>(ws);
+    a.xs = xs;
     const int threads = 64 * c.wy;
-    e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds);
-    if (e != hipSuccess) return hip_check(e, "hipFuncSetAttribute(stream LDS)");
-    int per_cu = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, c.lds);
-    if (e != hipSuccess) return hip_check(e, "occupancy query");
-    if (per_cu < 1) return fail(ST_ELDS, "streaming kernel does not fit on a CU");
-    int dev = 0, cus = 0;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    int grid = per_cu * cus;
-    if (grid > a.ntiles) grid = a.ntiles;
+    int grid = a.ntiles;
+    for (const void *fn : fns) {
+        e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds);
+        if (e != hipSuccess) return hip_check(e, "hipFuncSetAttribute(stream LDS)");
+        int per_cu = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, c.lds);
+        if (e != hipSuccess) return hip_check(e, "occupancy query");
+        if (per_cu < 1) return fail(ST_ELDS, "streaming kernel does not fit on a CU");
+        int dev = 0, cus = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        grid = std::min(grid, per_cu * cus);
+    }
     float kf = c.op == OP_THESIS ? (float)(2 * c.dims * c.r) : (float)c.weight;
     void *args[] = {&u1, &u2, &a, &kf};
-    if (!c.per_pass) {
-        // one cooperative launch: different shells run one sweep per pass
-        a.p0 = 0;
-        a.p1 = c.iters;
-        a.fuse_diff = 0;
-        e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, c.lds, s);
-        if (e != hipSuccess) return hip_check(e, "hipLaunchCooperativeKernel(stream)");
-        return ST_OK;
+    // Which kernel runs: outside graph capture (and outside the no-alloc
+    // entry point) the host reads the comparison flags once and launches
+    // only the kernel for the case that holds; otherwise both are launched
+    // and the device decides (the other returns at once, a few
+    // microseconds per launch).
+    int k_lo = 0, k_hi = 2;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (!workspace_is_fixed() && hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
+        int flags[kShellBlocks];
+        e = hipMemcpyAsync(flags, ws, sizeof flags, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_check(e, "shell comparison read-back");
+        int same = 1;
+        for (int i = 0; i < kShellBlocks; ++i) same &= flags[i] != 0;
+        k_lo = same ? 0 : 1;
+        k_hi = k_lo + 1;
     }
-    // one launch per pass, kernel boundaries as the barrier; n2 fused passes
-    // whatever the shells, each followed by the band fix-up (which returns
-    // at once when the shells are equal), then n1 single-sweep passes
-    a.fuse_diff = 1;
-    const FixArgs f = fix_args(c);
-    const void *fix = c.op == OP_THESIS ? reinterpret_cast<const void *>(&k_stream_fix<OP_THESIS>)
-                                        : reinterpret_cast<const void *>(&k_stream_fix<OP_JACOBI>);
-    long long fix_blocks = (f.total + 255) / 256;
-    if (fix_blocks > 1024) fix_blocks = 1024;
-    if (fix_blocks < 1) fix_blocks = 1;
-    const float *s0 = static_cast<const float *>(u1), *s1 = static_cast<const float *>(u2);
-    const int *flags = a.shell_eq;
+    if (!c.per_pass) {
+        // one cooperative launch per kernel, a grid barrier between passes
+        a.p0 = 0;
+        a.p1 = a.n2 + a.n1;
+        bool coop = true;
+        for (int k = k_lo; k < k_hi && coop; ++k) {
+            e = hipLaunchCooperativeKernel(fns[k], dim3(grid), dim3(threads), args, c.lds, s);
+            if (e == hipErrorIllegalState && k == k_lo) {
+                // the device takes no cooperative launch from this process
+                // now: the same passes follow, one launch per pass
+                (void)hipGetLastError();
+                coop = false;
+            } else if (e != hipSuccess) {
+                return hip_check(e, "hipLaunchCooperativeKernel(stream)");
+            }
+        }
+        if (coop) return ST_OK;
+    }
+    // one launch per pass and kernel, kernel boundaries as the barrier: n2
+    // fused passes, then n1 single-sweep passes (the kernel for different
+    // shells runs them too, when that is the case)
     for (int p = 0; p < a.n2 + a.n1; ++p) {
         a.p0 = p;
         a.p1 = p + 1;
-        e = hipLaunchKernel(fn, dim3(grid), dim3(threads), args, c.lds, s);
-        if (e != hipSuccess) return hip_check(e, "stream pass launch");
-        if (p < a.n2) {
-            const float *x = static_cast<const float *>((p & 1) ? u2 : u1);
-            float *y = static_cast<float *>((p & 1) ? u1 : u2);
-            FixArgs fa = f;
-            void *fargs[] = {&x, &s0, &s1, &y, &fa, &flags, &kf};
-            e = hipLaunchKernel(fix, dim3((unsigned)fix_blocks), dim3(256), fargs, 0, s);
-            if (e != hipSuccess) return hip_check(e, "stream band fix-up launch");
+        for (int k = k_lo; k < k_hi; ++k) {
+            e = hipLaunchKernel(fns[k], dim3(grid), dim3(threads), args, c.lds, s);
+            if (e != hipSuccess) return hip_check(e, "stream pass launch");
         }
     }
     return ST_OK;

@@ -380,13 +380,21 @@ st_status run_tblock(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2
     int grid = per_cu * cus;
     if (grid > a.ntiles) grid = a.ntiles;
     float kf = c.op == OP_THESIS ? (float)(2 * c.dims * c.r) : (float)c.weight;
-    if (!c.per_pass) {
+    bool coop = !c.per_pass;
+    if (coop) {
         a.pass0 = 0;
         a.pass1 = a.npass;
         void *args[] = {&u1, &u2, &a, &kf};
         err = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, c.lds, s);
-        if (err != hipSuccess) return hip_check(err, "hipLaunchCooperativeKernel(tblock)");
-    } else {
+        if (err == hipErrorIllegalState) {
+            // no cooperative launch from this process now: one launch per pass
+            (void)hipGetLastError();
+            coop = false;
+        } else if (err != hipSuccess) {
+            return hip_check(err, "hipLaunchCooperativeKernel(tblock)");
+        }
+    }
+    if (!coop) {
         for (int p = 0; p < a.npass; ++p) {
             a.pass0 = p;
             a.pass1 = p + 1;

@@ -358,6 +358,7 @@ def rocprof_metrics(cmd: List[str], workdir: str, timeout_s: int = 120) -> Dict[
     averaged over the sweep kernels' dispatches."""
     import glob
     import shutil
+    import signal
     import subprocess
 
     env = dict(os.environ, TMPDIR="/tmp")
@@ -366,12 +367,20 @@ def rocprof_metrics(cmd: List[str], workdir: str, timeout_s: int = 120) -> Dict[
         d = os.path.join(workdir, f"pass{i}")
         shutil.rmtree(d, ignore_errors=True)
         prof = ["rocprofv3", "--pmc", *counters, "--output-format", "csv", "-d", d, "-o", "p", "--", *cmd]
-        r = subprocess.run(prof, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
-                           timeout=timeout_s)
+        # its own process group, so that a pass over its time limit is ended
+        # with the program it profiles
+        proc = subprocess.Popen(prof, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                start_new_session=True)
+        try:
+            stdout, _ = proc.communicate(timeout=timeout_s)
+        except subprocess.TimeoutExpired:
+            os.killpg(proc.pid, signal.SIGKILL)
+            proc.communicate()
+            raise RuntimeError(f"rocprofv3 pass {counters} exceeded {timeout_s} s")
         files = glob.glob(os.path.join(d, "**", "p_counter_collection.csv"), recursive=True)
         if not files:
-            raise RuntimeError(f"rocprofv3 pass {counters} wrote no counters (exit {r.returncode}): "
-                               f"{r.stdout.decode(errors='replace')[-400:]}")
+            raise RuntimeError(f"rocprofv3 pass {counters} wrote no counters (exit {proc.returncode}): "
+                               f"{stdout.decode(errors='replace')[-400:]}")
         got.update(_read_counters(files[0]))
 
     def mean(name, idx=0):
