@@ -194,6 +194,20 @@ st_status stencil_sweep(const st_params *p, const void *d_in, void *d_out, void 
 st_status stencil_dispatch(const st_params *p, void *d_u1, void *d_u2, void *stream,
                            int *result_in_u2, float *elapsed_ms);
 
+/* stencil_dispatch without any allocation, for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation).  No counterpart in the reference.  The same
+ * sweeps and result as stencil_dispatch (elapsed time is not measured).  The
+ * dispatch's device scratch comes from a static device array of 16 MiB +
+ * 5 KiB (one per device, held until the process ends) instead of the
+ * library's allocated workspace: the row form's shell flag (4 KiB part) and
+ * the streaming form's shell flags and x-shell table (1 KiB + 16 B per grid
+ * row: grids with ny*nz up to 2^20); a larger request fails with ST_EINVAL.
+ * Calls that overlap on one device share that array and race; calls ordered
+ * on one stream, or one after another, do not. */
+st_status stencil_dispatch_noalloc(const st_params *p, void *d_u1, void *d_u2, void *stream,
+                                   int *result_in_u2);
+
 /* Write the initial condition `kind` (st_init) into d_u1 and d_u2 (both
  * nx*ny*nz elements of p->dtype, F32 or F64). */
 st_status stencil_init(const st_params *p, int kind, uint64_t seed, void *d_u1, void *d_u2,

@@ -40,6 +40,7 @@ SOURCES = [
     "unet.hip",
     "init_reduce.hip",
     "mixed.hip",
+    "noalloc.hip",
 ]
 
 
