@@ -18,7 +18,8 @@ one JSON line.
              storage with defect correction; errors against the FP64 CPU
              oracle, whose run is that leg's CPU baseline) and batched (512
              uneven grids, 50 sweeps, one graph launch per sweep against
-             per-problem launches); --no-legs skips them
+             per-problem launches) and reference_sizes (the reference's own
+             evaluation sizes 1024^3 and 2D 32768^2); --no-legs skips them
   warm_start the metric's second half: time to relative residual 1e-6 at
              128^3, learned warm start over plain zero start, on two sets of
              five problems (--no-warm skips it)
@@ -312,6 +313,37 @@ def mixed_leg(st, np, repeats: int = 3, with_cpu: bool = True):
     return out
 
 
+def size_legs(st, np, repeats: int = 3):
+    """The reference's own evaluation sizes (scripts/evaluate_stencils.py:71-76):
+    3D 1024^3 and 2D 32768^2, R=1 FP32 JACOBI, seeded boundary, 20 sweeps
+    each, AUTO's choice of kernel; GCells/s and the per-sweep algorithmic
+    HBM fraction (4 B x (cells + interior cells) per sweep)."""
+    out = {}
+    for name, dims, n in (("sweep1024", 3, 1024), ("sweep2d_32768", 2, 32768)):
+        its = 20
+        p = st.StencilParams.make(nx=n, ny=n, nz=n if dims == 3 else 1, dimensions=dims, radius=1,
+                                  iterations=its, op=st.OP_JACOBI, variant="auto", heuristic=1)
+        cfg = st.launch_config(p)
+        a = st.DeviceArray(p.shape, np.float32)
+        b = st.DeviceArray(p.shape, np.float32)
+        st.initialize_region(p, a, b, st.INIT_SEEDED, seed=SEED)
+        st.dispatch_kernels(p, a, b)  # warm-up
+        ts = []
+        for _ in range(repeats):
+            _, ms = st.dispatch_kernels(p, a, b, timed=True)
+            ts.append(ms)
+        a.free()
+        b.free()
+        ms = _median(ts)
+        cells = n ** dims
+        interior = (n - 2) ** dims
+        out[name] = {"workload": f"{dims}D {n}^{dims} R=1 FP32 JACOBI, {its} sweeps",
+                     "ms": round(ms, 3), "gcells_per_s": round(interior * its / ms / 1e6, 1),
+                     "hbm_frac": round(4 * (cells + interior) * its / (ms / 1e3) / 8e12, 4),
+                     "variant": {v: k for k, v in st.VARIANTS.items()}[cfg["variant"]], "launch": cfg}
+    return out
+
+
 def probe(st, np):
     d1 = st.DeviceArray((N, N, N), np.float32)
     d2 = st.DeviceArray((N, N, N), np.float32)
@@ -498,7 +530,8 @@ def main():
     if headline and not args.no_legs:
         reps = max(1, min(args.steps, 3))
         out["legs"] = {"mixed384": mixed_leg(st, np, reps, with_cpu=not args.no_cpu),
-                       "batched": batched_leg(st, np, reps)}
+                       "batched": batched_leg(st, np, reps),
+                       "reference_sizes": size_legs(st, np, reps)}
     if headline and not args.no_warm:
         out["warm_start"] = warm_start_leg()
     if not args.no_cpu:

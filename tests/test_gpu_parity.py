@@ -232,6 +232,8 @@ MIXED_SHAPES = [
     # neighbours across the wave seam exchanged through LDS (the layout the
     # 384^3 workload runs)
     (3, (300, 20, 18), 4), (3, (264, 16, 14), 1), (3, (384, 24, 20), 4), (3, (520, 14, 12), 2),
+    # rows of 256 and 512 columns: the consecutive layout (4 and 8 cells per lane)
+    (3, (256, 18, 14), 2), (3, (512, 12, 10), 3),
     # nx not a multiple of 4: the per-cell 3D kernels
     (3, (42, 33, 28), 1), (3, (42, 31, 27), 4),
 ]
@@ -444,6 +446,11 @@ STREAM_SHAPES = [
     ((128, 70, 61), {"block_z": 17, "heuristic": 0}),  # explicit z chunk
     ((512, 60, 40), {"coarsen_y": 2, "block_y": 8, "heuristic": 0}),   # 2 rows of 8 cells
     ((200, 45, 21), {"coarsen_y": 4, "block_y": 4, "heuristic": 0}),   # 4 rows of 4 cells
+    # over 512 columns: one row of 16 cells per lane
+    ((1024, 22, 14), {}),
+    ((1000, 17, 9), {}),                      # last quads past the row
+    ((516, 13, 11), {}),                      # second half of the lanes mostly idle
+    ((1024, 40, 12), {"coarsen_y": 1, "block_y": 6, "heuristic": 0}),  # 6 waves, 2 core rows
 ]
 
 
@@ -515,7 +522,7 @@ def test_coop_stream_one_shell_cell_differs(gpu, oracle, face, per_pass):
 
 def test_coop_stream_form_rejects_unsupported(gpu):
     st = gpu
-    for kw in (dict(nx=62, ny=40, nz=40), dict(nx=520, ny=40, nz=40), dict(dim=40, radius=2),
+    for kw in (dict(nx=62, ny=40, nz=40), dict(nx=1028, ny=40, nz=40), dict(dim=40, radius=2),
                dict(dim=40, dtype=st.F64)):
         p = st.StencilParams.make(variant="coop", temporal_form=3, heuristic=1, op=1,
                                   iterations=4, **kw)
@@ -597,6 +604,18 @@ ROWS_CASES = [
     ((640, 17, 11), "f32", (2,), {}),                # 3 waves per row, a short last part
     ((384, 21, 19), "f64", (4,), {}),                # frames shorter than 2R: two halo strips per wave
     ((1024, 18, 12), "f32", (3, 4), {}),             # 4 waves per row, two halo strips per wave
+    # consecutive layout (rows of 64*CX columns, CX consecutive cells per
+    # lane): 4, 6 and 8 cells per lane, both precisions, every radius whose
+    # kernel is built; F64 R=4 at 384 and R>1 at 512 take the quad layout
+    ((256, 22, 17), "f32", (1, 2, 3, 4), {}),
+    ((512, 14, 11), "f32", (1, 2, 3, 4), {}),
+    ((256, 19, 15), "f64", (1, 2, 3, 4), {}),
+    ((384, 23, 16), "f64", (1, 2, 3), {}),
+    ((512, 13, 10), "f64", (1, 2), {}),
+    ((384, 9, 30), "f32", (4,), {}),                 # one frame holds every row (9 of 12 waves)
+    # the quad layout chosen explicitly at 384 columns (8 cells per lane,
+    # 48 lanes busy)
+    ((384, 30, 24), "f32", (2, 4), {"coarsen_x": 8, "block_y": 8, "block_z": 24, "heuristic": 0}),
 ]
 
 
@@ -629,6 +648,21 @@ def test_smem_register_rows_form(gpu, oracle, shape, dt, radii, knobs, op):
             assert in_u2 == bool(its % 2)
             assert_bitwise(got, want, f"rows form {shape} {dt} r={r} op={op} its={its}")
             assert_bitwise(got_other, other, f"rows form other buffer {shape} {dt} r={r} op={op} its={its}")
+
+
+@pytest.mark.parametrize("nx,dt,r,cx", [(384, "f32", 4, 6), (256, "f32", 2, 4), (512, "f32", 3, 8),
+                                         (384, "f64", 3, 6), (384, "f64", 4, 4)])
+def test_rows_layout_choice(gpu, nx, dt, r, cx):
+    """Rows of 64*CX columns take the consecutive layout where its kernel is
+    built (coarsen_x = nx/64); F64 R=4 at 384 columns needs more registers
+    than that kernel may use and takes the quad layout (4 cells per lane,
+    two waves per row)."""
+    st = gpu
+    p = st.StencilParams.make(nx=nx, ny=40, nz=40, radius=r, iterations=3, op=1,
+                              dtype=st.F64 if dt == "f64" else st.F32, variant="smem_register",
+                              temporal_form=2, heuristic=1)
+    cfg = st.launch_config(p)
+    assert cfg["coarsen_x"] == cx and cfg["block_x"] == nx
 
 
 def test_smem_register_rows_form_rejects_unsupported(gpu):
@@ -737,6 +771,8 @@ def test_coop_cooperative_launch_timed_on_null_stream(gpu, oracle, form):
     (3, 320, 100, "coop"),           # both fields above 192 MiB
     (3, 288, 100, "smem_register"),  # below: the row kernel
     (3, 512, 2, "smem_register"),    # too few sweeps for a fused pass
+    (3, 1024, 100, "coop"),          # one row of 16 cells per lane
+    (3, 768, 100, "smem_register"),  # a quarter of those lanes idle: the row kernel
     (2, 256, 64, "base"),
     (2, 4096, 64, "smem_register"),
 ])

@@ -180,6 +180,7 @@ static st_status resolve(const st_params *p, Cfg &c, bool mixed)
     c.kind = 0;
     c.wx = c.wy = 0;
     c.hs = 1;
+    c.rowc = 0;
     c.shell_eq = nullptr;
     c.zchunk = 0;
     c.lds = 0;
@@ -212,7 +213,7 @@ static st_status resolve(const st_params *p, Cfg &c, bool mixed)
         const long long es = c.dtype == ST_F64 ? 8 : 4;
         const long long stream_bytes = 192LL << 20;
         const long long small = c.dims == 3 ? (1LL << 20) : (1LL << 22);
-        if (c.iters >= 4 && (c.tsteps == 0 || c.tsteps == 2) && c.form == 0 && stream_eligible(c) &&
+        if (c.iters >= 4 && (c.tsteps == 0 || c.tsteps == 2) && c.form == 0 && stream_auto_ok(c) &&
             2 * cells * es > stream_bytes) {
             c.variant = ST_V_COOP;
             c.cx = 1;

@@ -30,6 +30,7 @@ struct Cfg {
     int kind;            // COOP form resolved
     int wx, wy;          // COOP register form: waves across x and y
     int hs;              // SMEM_REGISTER row form: halo strips per wave
+    int rowc;            // row form: 1 = consecutive columns per lane (k_rowc)
     const int *shell_eq; // device flag: both buffers' shells hold the same bits
                          // (row form stores whole rows then); null: unknown
 };
@@ -80,6 +81,7 @@ st_status launch_rows_mixed(const Cfg &c, int defect, bool f16, const void *in, 
 
 // Streaming temporal blocking, whole rows per wave (sweep_stream.hip).
 bool stream_eligible(const Cfg &c);
+bool stream_auto_ok(const Cfg &c);  // AUTO takes the form for this shape
 st_status resolve_stream(Cfg &c);
 st_status run_stream(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2);
 

@@ -38,6 +38,8 @@
 // defect reads FP32 x and writes round16(w*sum - x).
 #include <hip/hip_cooperative_groups.h>
 
+#include <type_traits>
+
 #include "st_device.hpp"
 #include "st_half.hpp"
 #include "st_internal.hpp"
@@ -227,7 +229,8 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
         for (int i = 0; i < HS; ++i)
             if (has_halo[i]) load_row(kin_lo, halo_off[i], hxo[i], ph[i]);
         // correction sweep: the defect at the cells a step completes (plane
-        // m-R) is loaded one step ahead, with the plane prefetch
+        // m-R), loaded in the step after the plane prefetch (loading it one
+        // step ahead measured slower: more registers, fewer waves)
         U4 dqn[H];
         auto load_defect = [&](int k) {
             if constexpr (MODE == RM_CORR) {
@@ -238,7 +241,6 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
                 }
             }
         };
-        load_defect(kin_lo - R);
 
         // planes in phases of R: the ring and the partial sums rotate
         // through their R slots (logical entry i of phase rp in slot
@@ -275,17 +277,17 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
             }
             const int k = m - R;
             const bool out_plane = store_row && k >= k_out_lo && k < k_out_hi;
-            U4 dq[H];
-            if constexpr (MODE == RM_CORR) {
-#pragma unroll
-                for (int h = 0; h < H; ++h) dq[h] = dqn[h];
-            }
             if (m + 1 < kin_hi) {
                 load_row(m + 1, own_off, xo, pf);
 #pragma unroll
                 for (int i = 0; i < HS; ++i)
                     if (has_halo[i]) load_row(m + 1, halo_off[i], hxo[i], ph[i]);
-                load_defect(k + 1);
+            }
+            U4 dq[H];
+            load_defect(k);
+            if constexpr (MODE == RM_CORR) {
+#pragma unroll
+                for (int h = 0; h < H; ++h) dq[h] = dqn[h];
             }
             __syncthreads();
 
@@ -454,6 +456,277 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
     }
 }
 
+// ---------------------------------------------------------------- consecutive layout
+//
+// Rows of nx = 64*CX columns (CX = 4, 6 or 8; 256, 384 or 512 columns) are
+// held by one wave with CX consecutive columns per lane, so every lane is
+// busy (the quad layout above leaves a quarter of the lanes idle at 384
+// columns).  Loads, stores and LDS accesses are vectors of V consecutive
+// values; the x neighbours at distance d <= R <= CX are the lane's own
+// values or the adjacent lane's, one DPP rotate per value (R per side); the
+// rotate's wrap at lanes 0 and 63 reaches only cells of the x shell, which
+// are never computed.  Everything else (LDS plane per step, halo strips, z
+// ring and partial sums in phases of R, the 16-bit modes) is as in k_rows.
+template <typename E, int V> struct VecN { typedef E type __attribute__((ext_vector_type(V))); };
+
+// values per vector access: the largest power of two dividing CX with at
+// most 16 bytes
+template <int CX, int ES>
+constexpr int rowc_vec() { return (CX % 8 == 0 && 8 * ES <= 16) ? 8 : (CX % 4 == 0 && 4 * ES <= 16) ? 4 : 2; }
+
+// VGPRs the lane needs, in 32-bit words: ring and partial sums 2R*CX
+// values, the plane, its prefetch, the halo prefetch, the sums, the x
+// neighbours and the vector temporaries about 6*CX, and 48 words of
+// addresses and loop state (fitted to the compiler's allocation of every
+// instantiation).  The workgroup size follows from it: 4, 3 or 2 waves per
+// SIMD (128, 168 or 256 VGPRs, kept with a margin); beyond 200 words the
+// kernel is not built and the quad layout serves.
+template <int R, int CX, typename T>
+constexpr int rowc_words() { return (2 * R + 6) * CX * (int)(sizeof(T) / 4) + 48; }
+
+template <int R, int CX, typename T>
+constexpr int rowc_max_threads()
+{
+    constexpr int w = rowc_words<R, CX, T>();
+    return w <= 128 ? 1024 : w <= 150 ? 768 : w <= 200 ? 512 : 0;
+}
+
+template <int OP, int R, int CX, typename T, int MODE, bool F16>
+__global__ __launch_bounds__((rowc_max_threads<R, CX, T>())) void k_rowc(
+    const void *__restrict__ in0, void *__restrict__ out0, const uint16_t *__restrict__ aux, RowsArgs a, T kc)
+{
+    static_assert(R >= 1 && R <= CX, "x neighbours come from the adjacent lane");
+    static_assert(MODE == RM_SWEEP || sizeof(T) == 4, "the 16-bit modes compute in FP32");
+    // storage type of the input field
+    typedef typename std::conditional<MODE == RM_CORR, uint16_t, T>::type S;
+    constexpr int VS = rowc_vec<CX, (int)sizeof(S)>();  // global loads
+    constexpr int VT = rowc_vec<CX, (int)sizeof(T)>();  // LDS and sweep stores
+    constexpr int VH = rowc_vec<CX, 2>();                // 16-bit stores and defect loads
+    typedef typename VecN<S, VS>::type SV;
+    typedef typename VecN<T, VT>::type TV;
+    typedef typename VecN<uint16_t, VH>::type HV;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T *lds = reinterpret_cast<T *>(lds_raw);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wy = a.wy;
+    constexpr int RS = 64 * CX;   // LDS row stride (values)
+    const int lrows = wy + 2 * R;  // LDS rows per buffer: R halo, wy core, R halo
+    const int xo = lane * CX;      // first column of the lane
+    const bool act = xo < a.nx;
+    const int xl = act ? xo : 0;   // load column (idle lanes read the row start)
+    const bool whole = xo >= R && xo + CX <= a.nx - R;  // every cell interior
+    const bool eqs = MODE == RM_SWEEP && a.eq != nullptr && *a.eq != 0;
+    // halo strip of this wave: waves 0..2R-1 each load one of the frame's
+    // 2R halo rows (rows -R..-1 into LDS rows 0..R-1, wy..wy+R-1 into
+    // R+wy..)
+    const bool has_halo = w < 2 * R;
+    const int hrow = w < R ? w : wy + w;
+    const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int nsweeps = MODE == RM_SWEEP ? a.sweeps : 1;
+
+    for (int sw = 0; sw < nsweeps; ++sw) {
+    const void *in_ = (sw & 1) ? static_cast<const void *>(out0) : in0;
+    void *out_ = (sw & 1) ? const_cast<void *>(in0) : out0;
+    if (sw > 0) cooperative_groups::this_grid().sync();
+    for (int tile = (int)bid; tile < a.ntiles; tile += (int)gridDim.x) {
+        const int tyi = tile % a.tiles_y, tzi = tile / a.tiles_y;
+        const int fy0 = tyi * wy;
+        const int gy = fy0 + w;
+        const int own_off = min(gy, a.ny - 1) * a.nx;
+        const bool store_row = gy >= R && gy < a.ny - R;
+        const int hgy = w < R ? fy0 - R + w : fy0 + wy + (w - R);
+        const int halo_off = min(max(hgy, 0), a.ny - 1) * a.nx;
+        const int z0 = tzi * a.zchunk, z1 = min(z0 + a.zchunk, a.nz);
+        const int k_out_lo = max(z0, R), k_out_hi = min(z1, a.nz - R);
+
+        auto load_row = [&](int k, int off, T (&d)[CX]) {
+            k = min(max(k, 0), a.nz - 1);
+            const S *p = static_cast<const S *>(in_) + (long long)k * a.sz + off + xl;
+#pragma unroll
+            for (int v = 0; v < CX / VS; ++v) {
+                const SV q = *reinterpret_cast<const SV *>(p + v * VS);
+#pragma unroll
+                for (int j = 0; j < VS; ++j) {
+                    if constexpr (MODE == RM_CORR) d[v * VS + j] = wid16<F16>(q[j]);
+                    else d[v * VS + j] = q[j];
+                }
+            }
+        };
+
+        T ring[R][CX], part[R][CX], pf[CX], ph[CX];
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int c = 0; c < CX; ++c) ring[i][c] = part[i][c] = (T)0;
+        const int kin_lo = z0 - R, kin_hi = z1 + R;
+        load_row(kin_lo, own_off, pf);
+        if (has_halo) load_row(kin_lo, halo_off, ph);
+        // correction sweep: the defect of the cells a step completes (plane
+        // m-R), loaded in the step after the plane prefetch
+        uint16_t dqn[CX];
+        auto load_defect = [&](int k) {
+            if constexpr (MODE == RM_CORR) {
+                if (store_row && k >= k_out_lo && k < k_out_hi) {
+                    const uint16_t *p = aux + (long long)k * a.sz + own_off + xl;
+#pragma unroll
+                    for (int v = 0; v < CX / VH; ++v) {
+                        const HV q = *reinterpret_cast<const HV *>(p + v * VH);
+#pragma unroll
+                        for (int j = 0; j < VH; ++j) dqn[v * VH + j] = q[j];
+                    }
+                }
+            }
+        };
+
+        for (int m0 = kin_lo; m0 < kin_hi; m0 += R) {
+#pragma unroll
+        for (int rp = 0; rp < R; ++rp) {
+            const int m = m0 + rp;
+            if (m >= kin_hi) break;
+            __builtin_amdgcn_sched_barrier(0);
+            T nw[CX];
+#pragma unroll
+            for (int c = 0; c < CX; ++c) nw[c] = pf[c];
+            T *buf = lds + (size_t)((m & 1) * lrows) * RS;
+#pragma unroll
+            for (int v = 0; v < CX / VT; ++v) {
+                TV q;
+#pragma unroll
+                for (int j = 0; j < VT; ++j) q[j] = nw[v * VT + j];
+                *reinterpret_cast<TV *>(buf + (R + w) * RS + xo + v * VT) = q;
+            }
+            if (has_halo) {
+#pragma unroll
+                for (int v = 0; v < CX / VT; ++v) {
+                    TV q;
+#pragma unroll
+                    for (int j = 0; j < VT; ++j) q[j] = ph[v * VT + j];
+                    *reinterpret_cast<TV *>(buf + hrow * RS + xo + v * VT) = q;
+                }
+            }
+            const int k = m - R;
+            const bool out_plane = store_row && k >= k_out_lo && k < k_out_hi;
+            if (m + 1 < kin_hi) {
+                load_row(m + 1, own_off, pf);
+                if (has_halo) load_row(m + 1, halo_off, ph);
+            }
+            uint16_t dq[CX];
+            load_defect(k);
+            if constexpr (MODE == RM_CORR) {
+#pragma unroll
+                for (int c = 0; c < CX; ++c) dq[c] = dqn[c];
+            }
+            __syncthreads();
+
+            // x terms: own values, else the adjacent lane's (pv[i] holds
+            // column xo-R+i, nv[i] column xo+CX+i)
+            T pv[R], nv[R];
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                pv[i] = dpp_prev(nw[CX - R + i]);
+                nv[i] = dpp_next(nw[i]);
+            }
+            T s[CX];
+#pragma unroll
+            for (int c = 0; c < CX; ++c) {
+                T u = (T)0;
+#pragma unroll
+                for (int d = R; d >= 1; --d) u += c - d >= 0 ? nw[c - d] : pv[R + c - d];
+#pragma unroll
+                for (int d = 1; d <= R; ++d) u += c + d < CX ? nw[c + d] : nv[c + d - CX];
+                s[c] = u;
+            }
+            // y terms from the LDS plane: rows R+w-d (d = R..1), then R+w+d
+#pragma unroll
+            for (int d = R; d >= 1; --d) {
+#pragma unroll
+                for (int v = 0; v < CX / VT; ++v) {
+                    const TV q = *reinterpret_cast<const TV *>(buf + (R + w - d) * RS + xo + v * VT);
+#pragma unroll
+                    for (int j = 0; j < VT; ++j) s[v * VT + j] += q[j];
+                }
+            }
+#pragma unroll
+            for (int d = 1; d <= R; ++d) {
+#pragma unroll
+                for (int v = 0; v < CX / VT; ++v) {
+                    const TV q = *reinterpret_cast<const TV *>(buf + (R + w + d) * RS + xo + v * VT);
+#pragma unroll
+                    for (int j = 0; j < VT; ++j) s[v * VT + j] += q[j];
+                }
+            }
+            // k-prev terms: logical ring[i] = plane m-R+i, in slot (rp+i) % R
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+#pragma unroll
+                for (int c = 0; c < CX; ++c) s[c] += ring[(rp + i) % R][c];
+            // plane m is the next k-next term of P_{m-R} .. P_{m-1}
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+#pragma unroll
+                for (int c = 0; c < CX; ++c) part[(rp + i) % R][c] += nw[c];
+            // P_{m-R} (slot rp) is complete: plane k, centre ring slot rp
+            if (out_plane && act) {
+                if constexpr (MODE == RM_SWEEP) {
+                    T res[CX];
+#pragma unroll
+                    for (int c = 0; c < CX; ++c) res[c] = finish<OP>(part[rp][c], ring[rp][c], kc);
+                    T *row = static_cast<T *>(out_) + (long long)k * a.sz + (long long)gy * a.nx + xo;
+                    if (whole || eqs) {
+                        // equal shells: shell cells with the value they
+                        // hold, so rows go out as whole lines
+#pragma unroll
+                        for (int v = 0; v < CX / VT; ++v) {
+                            TV q;
+#pragma unroll
+                            for (int j = 0; j < VT; ++j) {
+                                const int c = v * VT + j;
+                                q[j] = whole || interior1(xo + c, a.nx, R) ? res[c] : ring[rp][c];
+                            }
+                            *reinterpret_cast<TV *>(row + v * VT) = q;
+                        }
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < CX; ++c)
+                            if (interior1(xo + c, a.nx, R)) row[c] = res[c];
+                    }
+                } else {
+                    // 16-bit output: correction round16(widen(d) + w*sum),
+                    // defect round16(w*sum - x); the shells of the 16-bit
+                    // outputs are zero, so whole vectors go out
+                    uint16_t res[CX];
+#pragma unroll
+                    for (int c = 0; c < CX; ++c) {
+                        const float sw = part[rp][c] * kc;
+                        uint16_t r16;
+                        if constexpr (MODE == RM_CORR) r16 = rnd16<F16>(wid16<F16>(dq[c]) + sw);
+                        else r16 = rnd16<F16>(sw - ring[rp][c]);
+                        res[c] = whole || interior1(xo + c, a.nx, R) ? r16 : (uint16_t)0;
+                    }
+                    uint16_t *row = static_cast<uint16_t *>(out_) + (long long)k * a.sz + (long long)gy * a.nx + xo;
+#pragma unroll
+                    for (int v = 0; v < CX / VH; ++v) {
+                        HV q;
+#pragma unroll
+                        for (int j = 0; j < VH; ++j) q[j] = res[v * VH + j];
+                        *reinterpret_cast<HV *>(row + v * VH) = q;
+                    }
+                }
+            }
+            // plane m and P_m take the slot of plane m-R and P_{m-R}
+#pragma unroll
+            for (int c = 0; c < CX; ++c) {
+                ring[rp][c] = nw[c];
+                part[rp][c] = s[c];
+            }
+        }
+        }
+        __syncthreads();  // the next tile rewrites the LDS planes
+    }
+    }
+}
+
 // ---------------------------------------------------------------- host side
 
 // Built instantiations: every (R, CX, T) of the sweep whose registers fit,
@@ -514,6 +787,60 @@ static const void *rows_fn_mode(const Cfg &c, int cx, int mode, bool f16, int *t
     return rows_fn_d<float, RM_SWEEP, false>(c.op, c.r, cx, hs, threads);
 }
 
+// Consecutive layout: rows of exactly 64*CX columns, CX = 4, 6 or 8 and at
+// least the radius.
+static int rowc_cx(const Cfg &c)
+{
+    if (c.nx % 64 || c.r < 1 || c.r > 4) return 0;
+    const int cx = c.nx / 64;
+    return (cx == 4 || cx == 6 || cx == 8) && c.r <= cx ? cx : 0;
+}
+
+template <int R, int CX, typename T, int MODE, bool F16>
+static const void *rowc_fn_t(int op, int *threads)
+{
+    *threads = rowc_max_threads<R, CX, T>();
+    if constexpr (rowc_max_threads<R, CX, T>() == 0)
+        return nullptr;
+    else if constexpr (MODE != RM_SWEEP)
+        return reinterpret_cast<const void *>(&k_rowc<OP_JACOBI, R, CX, T, MODE, F16>);
+    else
+        return op == OP_THESIS ? reinterpret_cast<const void *>(&k_rowc<OP_THESIS, R, CX, T, MODE, F16>)
+                               : reinterpret_cast<const void *>(&k_rowc<OP_JACOBI, R, CX, T, MODE, F16>);
+}
+
+template <typename T, int MODE, bool F16>
+static const void *rowc_fn_d(int op, int r, int cx, int *threads)
+{
+#define RC_CASE(RR, CC) \
+    if (r == RR && cx == CC) return rowc_fn_t<RR, CC, T, MODE, F16>(op, threads);
+    RC_CASE(1, 4) RC_CASE(2, 4) RC_CASE(3, 4) RC_CASE(4, 4)
+    RC_CASE(1, 6) RC_CASE(2, 6) RC_CASE(3, 6) RC_CASE(4, 6)
+    RC_CASE(1, 8) RC_CASE(2, 8) RC_CASE(3, 8) RC_CASE(4, 8)
+#undef RC_CASE
+    return nullptr;
+}
+
+static const void *rowc_fn_mode(const Cfg &c, int cx, int mode, bool f16, int *threads)
+{
+    *threads = 0;
+    if (mode == RM_CORR)
+        return f16 ? rowc_fn_d<float, RM_CORR, true>(c.op, c.r, cx, threads)
+                   : rowc_fn_d<float, RM_CORR, false>(c.op, c.r, cx, threads);
+    if (mode == RM_DEFECT)
+        return f16 ? rowc_fn_d<float, RM_DEFECT, true>(c.op, c.r, cx, threads)
+                   : rowc_fn_d<float, RM_DEFECT, false>(c.op, c.r, cx, threads);
+    if (c.dtype == ST_F64) return rowc_fn_d<double, RM_SWEEP, false>(c.op, c.r, cx, threads);
+    return rowc_fn_d<float, RM_SWEEP, false>(c.op, c.r, cx, threads);
+}
+
+// The row kernel a resolved configuration launches.
+static const void *rows_kernel(const Cfg &c, int mode, bool f16, int *threads)
+{
+    if (c.rowc) return rowc_fn_mode(c, c.cx, mode, f16, threads);
+    return rows_fn_mode(c, c.cx, mode, f16, threads, c.hs);
+}
+
 // Lane layout: 4 cells per lane and one wave per row up to 256 columns; up
 // to 512, 8 cells per lane for sweeps where that kernel is built (measured
 // on 384^3 and 512^3: equal or faster at every radius, up to 1.19x at R=3)
@@ -564,7 +891,18 @@ static st_status rows_resolve_mode(Cfg &c, int mode, bool f16)
 {
     int cx = 0, wx = 1, threads = 0, hs = 1;
     const void *fn = nullptr;
-    if (c.dims != 3 || !rows_layout(c, mode, f16, &cx, &wx, &threads, &fn, &hs))
+    // Rows of 256, 384 or 512 columns take the consecutive layout (the
+    // library's choice, or coarsen_x = nx/64 with HEURISTIC=0): every lane
+    // busy, one DPP rotate per neighbour value, no selects.
+    const int ccx = c.dims == 3 ? rowc_cx(c) : 0;
+    const bool consec = ccx && (c.heuristic || c.cx == ccx);
+    c.rowc = 0;
+    if (consec) {
+        cx = ccx;
+        fn = rowc_fn_mode(c, cx, mode, f16, &threads);
+        if (fn && threads / 64 >= 2 * c.r) c.rowc = 1;
+    }
+    if (!c.rowc && (c.dims != 3 || !rows_layout(c, mode, f16, &cx, &wx, &threads, &fn, &hs)))
         return fail(ST_EUNSUPPORTED, "row-streaming form needs a 3D F32/F64 grid, radius 1..4 and nx a "
                                      "multiple of 4 whose rows fit a workgroup with its halo strips");
     const int rows_max = threads / 64 / wx;
@@ -580,7 +918,8 @@ static st_status rows_resolve_mode(Cfg &c, int mode, bool f16)
         wy = c.ny < rows_min ? rows_min : c.ny;
     }
     const int es = (c.dtype == ST_F64 && mode == RM_SWEEP) ? 8 : 4;
-    auto lds_of = [&](int rows) { return 2 * (rows + 2 * c.r) * wx * (cx / 4) * 64 * 4 * es; };
+    // LDS: two planes of (rows + 2R) rows of wx*64 lanes' cells
+    auto lds_of = [&](int rows) { return 2 * (rows + 2 * c.r) * wx * cx * 64 * es; };
     // the heuristic frame shrinks until its double-buffered planes fit
     while ((c.heuristic || c.by <= 0) && lds_of(wy) > kMaxLds && wy > rows_min) --wy;
     c.lds = lds_of(wy);
@@ -692,7 +1031,7 @@ static st_status rows_launch(const Cfg &c, const void *fn, const void *in, void 
 st_status launch_rows(const Cfg &c, const void *in, void *out, hipStream_t s)
 {
     int threads = 0;
-    const void *fn = rows_fn_mode(c, c.cx, RM_SWEEP, false, &threads, c.hs);
+    const void *fn = rows_kernel(c, RM_SWEEP, false, &threads);
     if (!fn || 64 * c.wx * c.wy > threads) return fail(ST_EUNSUPPORTED, "row-streaming kernel missing");
     return rows_launch(c, fn, in, out, nullptr, s);
 }
@@ -718,7 +1057,7 @@ st_status run_rows_coop(const Cfg &c0, void *u1, void *u2, hipStream_t s, int *i
     if (c0.iters <= 0) return ST_OK;
     Cfg c = c0;
     int threads = 0;
-    const void *fn = rows_fn_mode(c, c.cx, RM_SWEEP, false, &threads, c.hs);
+    const void *fn = rows_kernel(c, RM_SWEEP, false, &threads);
     if (!fn || 64 * c.wx * c.wy > threads) return fail(ST_EUNSUPPORTED, "row-streaming kernel missing");
     st_status st = ST_OK;
     if (c.iters > 1) st = rows_shell_check(c, u1, u2, s, &c.shell_eq);
@@ -804,7 +1143,7 @@ st_status launch_rows_mixed(const Cfg &c, int defect, bool f16, const void *in, 
     st_status st = rows_resolve_mode(rc, mode, f16);
     if (st != ST_OK) return st;
     int threads = 0;
-    const void *fn = rows_fn_mode(rc, rc.cx, mode, f16, &threads, rc.hs);
+    const void *fn = rows_kernel(rc, mode, f16, &threads);
     if (!fn || 64 * rc.wx * rc.wy > threads) return fail(ST_EUNSUPPORTED, "row-streaming mixed kernel missing");
     return rows_launch(rc, fn, in, out, d, s);
 }
