@@ -130,19 +130,17 @@ typedef struct st_params {
                              2 = register-queue full-row frames (F32, radius
                              1, temporal_steps 2, even iteration count),
                              3 = streaming rows, one wave per grid row (F32,
-                             radius 1, nx a multiple of 4 up to 512; two
-                             fused sweeps per pass; as one cooperative launch
-                             only when both buffers hold the same shell (else
-                             one sweep per pass); with launch_per_pass one
-                             launch per pass whatever the shells, a band
-                             fix-up kernel after each fused pass),
+                             radius 1, nx a multiple of 4 up to 1024; two
+                             fused sweeps per pass whether or not the two
+                             buffers hold the same shell; block_y = waves
+                             per frame, coarsen_y = rows per wave),
                              4 = row frames of the SMEM_REGISTER row form, one
                              sweep per grid barrier (3D F32/F64, radius 1..4,
                              nx a multiple of 4).
                              SMEM_REGISTER: 0 = library's choice, 1 = register
                              march, 2 = row streaming (3D F32/F64, radius
-                             1..4, nx a multiple of 4 up to 512; block_y =
-                             rows per frame, block_z = z-chunk depth)       */
+                             1..4, nx a multiple of 4; block_y = rows per
+                             frame, block_z = z-chunk depth)                */
     double weight;        /* JACOBI neighbour weight (0 = 1/(2*D*R)); rounded
                              to the storage type                               */
 } st_params;

@@ -22,10 +22,10 @@
 //     pass moves the same HBM bytes as with equal shells.  Only interior
 //     cells are stored, so both shells stay untouched.
 //
-// Layout: a workgroup of WY waves owns a frame of 2*WY full rows (a wave
-// holds two rows, each lane CX cells of both, in quads of four consecutive
-// columns 256 apart) and marches it
-// along z over a chunk of planes.  Per level and plane:
+// Layout: a workgroup of WY waves owns a frame of RPW*WY full rows (a wave
+// holds RPW rows, each lane CX cells of each, in quads of four consecutive
+// columns 256 apart: two rows of 4 or 8 cells up to 512 columns, one row of
+// 16 cells up to 1024) and marches it along z over a chunk of planes.  Per level and plane:
 //   x neighbours   own registers, lane edges by DPP wave shifts (the row
 //                  fits one wave, so the only missing neighbours are the
 //                  ones outside the grid, next to shell cells);
@@ -63,9 +63,12 @@ struct SmArgs {
 
 // Index of the x-shell value of buffer b (0: u1, 1: u2) at row j of plane
 // k, column 0 (e = 0) or nx-1 (e = 1), in the table k_shell_eq writes.
+// Planes are contiguous per row: a wave marching z reads one 128-byte line
+// per 16 planes of each of its rows (plane-major order measured 20% slower
+// with different shells: a new line, mostly an HBM miss, every plane).
 __device__ __forceinline__ long long xshell_index(int b, int k, int j, int e, int ny, int nz)
 {
-    return (((long long)b * nz + k) * ny + j) * 2 + e;
+    return (((long long)b * ny + j) * nz + k) * 2 + e;
 }
 
 // part[b] = 1 when every shell cell block b visits holds the same bits in
@@ -356,6 +359,11 @@ __device__ __forceinline__ void sm_finish(SmPlane<CX, RPW> &out, const SmPlane<C
 #pragma unroll
                     for (int j = 0; j < 4; ++j) out.v[r][4 * h + j] = q[j];
                 }
+            // wait for these loads inside this rare branch: the plane shares
+            // its registers with the computed one, and a wait placed after
+            // the join would hold every step until the prefetch of the next
+            // plane has landed (vmcnt(0), expcnt and lgkmcnt untouched)
+            __builtin_amdgcn_s_waitcnt(0x0F70);
         } else {
             out = cen;
         }
@@ -504,10 +512,12 @@ __device__ __forceinline__ void sm_pass(const float *__restrict__ X, float *__re
 // values plus the plane in flight): 8 values fit the 128 VGPRs of 16 waves,
 // 16 the 168 of 12 waves, 32 the 256 of 8 waves.  The kernel for different
 // shells (DIFF) also carries the shell values that travel with two of the
-// planes: 8 values fit 12 waves, 16 and more 8 waves.
+// planes: 8 values fit 12 waves, 16 and more 8 waves.  One row of 16 cells
+// (per-row temporaries of 16 values) takes 8 waves in both kernels.
 template <int CX, int RPW, bool DIFF>
 constexpr int sm_max_threads()
 {
+    if (CX > 8) return 512;
     return DIFF ? (CX * RPW <= 8 ? 768 : 512) : (CX * RPW <= 8 ? 1024 : CX * RPW <= 16 ? 768 : 512);
 }
 
@@ -540,11 +550,18 @@ __global__ __launch_bounds__((sm_max_threads<CX, RPW, DIFF>())) void k_stream(fl
 
 // ---------------------------------------------------------------- host side
 
+// Cells per lane: 4 up to 256 columns, 8 up to 512, 16 up to 1024 (one row
+// per wave then).
 static int stream_cx(const Cfg &c)
 {
-    if (c.nx % 4 || c.nx > 512) return 0;
-    return c.nx <= 256 ? 4 : 8;
+    if (c.nx % 4 || c.nx > 1024) return 0;
+    return c.nx <= 256 ? 4 : c.nx <= 512 ? 8 : 16;
 }
+
+// AUTO's use of the form: one row of 16 cells per lane leaves a quarter of
+// the lanes idle at 768 columns, where the row kernel measured faster (768^3:
+// 647 against 601 GCells/s); at 1024^3 the form gives 1010 against 697.
+bool stream_auto_ok(const Cfg &c) { return stream_eligible(c) && (c.nx <= 512 || c.nx > 896); }
 
 bool stream_eligible(const Cfg &c)
 {
@@ -559,10 +576,12 @@ static const void *stream_fn_t(int op)
                            : reinterpret_cast<const void *>(&k_stream<OP_JACOBI, CX, RPW, DIFF>);
 }
 
-// Instantiated shapes: 8 cells x 2 rows (nx up to 512), 4 cells x 2 or 4
-// rows (nx up to 256).  8 x 4 needs more than 256 VGPRs and is not built.
+// Instantiated shapes: 16 cells x 1 row (nx up to 1024), 8 cells x 2 rows
+// (nx up to 512), 4 cells x 2 or 4 rows (nx up to 256).  8 x 4 needs more
+// than 256 VGPRs and is not built.
 static const void *stream_fn(int op, int cx, int rpw, bool diff)
 {
+    if (cx == 16) return diff ? stream_fn_t<16, 1, true>(op) : stream_fn_t<16, 1, false>(op);
     if (cx == 8) return diff ? stream_fn_t<8, 2, true>(op) : stream_fn_t<8, 2, false>(op);
     if (rpw == 4) return diff ? stream_fn_t<4, 4, true>(op) : stream_fn_t<4, 4, false>(op);
     return diff ? stream_fn_t<4, 2, true>(op) : stream_fn_t<4, 2, false>(op);
@@ -571,6 +590,7 @@ static const void *stream_fn(int op, int cx, int rpw, bool diff)
 // Waves per frame that both kernels of a shape take.
 static int stream_max_waves(int cx, int rpw)
 {
+    if (cx == 16) return std::min(sm_max_threads<16, 1, false>(), sm_max_threads<16, 1, true>()) / 64;
     if (cx == 8) return std::min(sm_max_threads<8, 2, false>(), sm_max_threads<8, 2, true>()) / 64;
     if (rpw == 4) return std::min(sm_max_threads<4, 4, false>(), sm_max_threads<4, 4, true>()) / 64;
     return std::min(sm_max_threads<4, 2, false>(), sm_max_threads<4, 2, true>()) / 64;
@@ -610,13 +630,15 @@ st_status resolve_stream(Cfg &c)
 {
     if (!stream_eligible(c))
         return fail(ST_EUNSUPPORTED, "streaming temporal form needs FP32, 3D, radius 1, nx a multiple "
-                                     "of 4 up to 512");
+                                     "of 4 up to 1024");
     const int cx = stream_cx(c);
-    // rows per wave: 2, or coarsen_y = 4 with 4 cells per lane (nx <= 256)
-    int rpw = 2;
+    // rows per wave: 2, or coarsen_y = 4 with 4 cells per lane (nx <= 256);
+    // one row of 16 cells per lane over 512 columns
+    int rpw = cx == 16 ? 1 : 2;
     if (!c.heuristic && c.cy > 0) {
-        if (c.cy != 2 && !(c.cy == 4 && cx == 4))
-            return fail(ST_EINVAL, "streaming form: coarsen_y must be 2, or 4 when nx <= 256");
+        if (cx == 16 ? c.cy != 1 : (c.cy != 2 && !(c.cy == 4 && cx == 4)))
+            return fail(ST_EINVAL, "streaming form: coarsen_y must be 2 (or 4 when nx <= 256; 1 when "
+                                   "nx > 512)");
         rpw = c.cy;
     }
     const int wmax = stream_max_waves(cx, rpw);
