@@ -160,29 +160,36 @@ struct SmTile {
 };
 
 // Shell values of a fused pass with different shells (unused with equal
-// shells).  SmShell0 travels with a prefetched level-0 plane: its x-shell
-// values from u1, which replace the streamed buffer's when the plane is
-// consumed.  SmShell1 holds the level-1 shell values of one plane from u2:
-// the x-shell values of each row and the grid's y-shell rows; it is loaded
-// one step before the step that finishes that plane.
-template <int RPW>
-struct SmShell0 {
-    float2 x0[RPW];     // u1 at columns 0 and nx-1
-};
+// shells).  The x-shell values (columns 0 and nx-1) of the tile's frame
+// rows, for every plane the tile touches and both buffers, are copied from
+// the table into LDS when the tile starts (SmXSlice); the pass reads them
+// there (level 0: u1's, patched in when a plane is consumed; level 1:
+// u2's).  Loading them from the table every plane instead (four small
+// loads per plane, with their address arithmetic) made the pass 20%
+// slower.  SmShell1 holds the grid's y-shell rows of u2 for the plane that
+// the next step finishes at level 1 (rows with ytrue only).
 template <int CX, int RPW>
 struct SmShell1 {
-    float2 x1[RPW];     // u2 at columns 0 and nx-1
     float y1[RPW][CX];  // u2, rows with ytrue
 };
 
+// LDS copy of the tile's x-shell values: entry (b, fr, kk) holds buffer b's
+// values at frame row fr (clamped into the grid) of plane z0-3+kk (clamped).
+struct SmXSlice {
+    float2 *v;
+    int rows;    // frame rows
+    int stride;  // entries per row: zchunk + 5
+    int k0;      // plane of entry 0: z0 - 3
+    __device__ float2 at(int b, int fr, int k) const { return v[(b * rows + fr) * stride + (k - k0)]; }
+};
+
 // Level-0 plane k (clamped into the grid) from X.  DIFF: shell planes and
-// shell rows from S0 (u1) instead, and the x-shell values of each row from
-// the table; they are patched in when the plane is consumed (sm_take), so
-// that the loads stay in flight meanwhile.
+// shell rows from S0 (u1) instead; the x-shell values of u1 are patched in
+// when the plane is consumed (sm_take).
 template <int CX, int RPW, bool DIFF>
-__device__ __forceinline__ void sm_load(SmPlane<CX, RPW> &d, SmShell0<RPW> &sh, const float *__restrict__ X,
-                                        const float *__restrict__ S0, const float *__restrict__ xs,
-                                        const SmArgs &a, const SmTile<RPW> &t, const SmLane<CX> &l, int k)
+__device__ __forceinline__ void sm_load(SmPlane<CX, RPW> &d, const float *__restrict__ X,
+                                        const float *__restrict__ S0, const SmArgs &a, const SmTile<RPW> &t,
+                                        const SmLane<CX> &l, int k)
 {
     typedef float V4 __attribute__((ext_vector_type(4)));
     k = min(max(k, 0), a.nz - 1);
@@ -200,25 +207,17 @@ __device__ __forceinline__ void sm_load(SmPlane<CX, RPW> &d, SmShell0<RPW> &sh, 
             d.v[r][4 * h + 3] = q[3];
         }
     }
-    if constexpr (DIFF) {
-#pragma unroll
-        for (int r = 0; r < RPW; ++r)
-            sh.x0[r] = *reinterpret_cast<const float2 *>(xs + xshell_index(0, k, t.rowc[r], 0, a.ny, a.nz));
-    }
 }
 
-// Level-1 shell values of plane k from u2 (S1): x-shell values from the
-// table, the y-shell rows from S1.
+// Level-1 y-shell rows of plane k from u2 (S1).
 template <int CX, int RPW>
 __device__ __forceinline__ void sm_load_sh1(SmShell1<CX, RPW> &sh, const float *__restrict__ S1,
-                                            const float *__restrict__ xs, const SmArgs &a, const SmTile<RPW> &t,
-                                            const SmLane<CX> &l, int k)
+                                            const SmArgs &a, const SmTile<RPW> &t, const SmLane<CX> &l, int k)
 {
     typedef float V4 __attribute__((ext_vector_type(4)));
     k = min(max(k, 0), a.nz - 1);
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
-        sh.x1[r] = *reinterpret_cast<const float2 *>(xs + xshell_index(1, k, t.rowc[r], 0, a.ny, a.nz));
         if (t.ytrue[r]) {
             const float *rb = S1 + (long long)k * a.sz + t.rowoff[r];
 #pragma unroll
@@ -231,20 +230,23 @@ __device__ __forceinline__ void sm_load_sh1(SmShell1<CX, RPW> &sh, const float *
     }
 }
 
-// The loaded plane with its level-0 x-shell cells replaced by u1's (DIFF).
+// The loaded plane k with its level-0 x-shell cells replaced by u1's
+// (DIFF; frame rows w*RPW + r).
 template <int CX, int RPW, bool DIFF>
-__device__ __forceinline__ void sm_take(SmPlane<CX, RPW> &d, const SmPlane<CX, RPW> &p,
-                                        const SmShell0<RPW> &sh, const SmLane<CX> &l)
+__device__ __forceinline__ void sm_take(SmPlane<CX, RPW> &d, const SmPlane<CX, RPW> &p, const SmXSlice &xsl,
+                                        int w, int k, const SmLane<CX> &l)
 {
     d = p;
     if constexpr (DIFF) {
 #pragma unroll
-        for (int r = 0; r < RPW; ++r)
+        for (int r = 0; r < RPW; ++r) {
+            const float2 x0 = xsl.at(0, w * RPW + r, k);
 #pragma unroll
             for (int h = 0; h < CX / 4; ++h) {
-                d.v[r][4 * h] = l.lo[h] ? sh.x0[r].x : d.v[r][4 * h];
-                d.v[r][4 * h + 3] = l.hi[h] ? sh.x0[r].y : d.v[r][4 * h + 3];
+                d.v[r][4 * h] = l.lo[h] ? x0.x : d.v[r][4 * h];
+                d.v[r][4 * h + 3] = l.hi[h] ? x0.y : d.v[r][4 * h + 3];
             }
+        }
     }
 }
 
@@ -333,14 +335,15 @@ __device__ __forceinline__ void sm_partial(SmPlane<CX, RPW> &out, const SmPlane<
 // out = finish(partial + z-next) with cells outside the interior taking the
 // centre value (equal shells: a shell cell never changes).  SH1 (level 1 of
 // a fused pass with different shells): shell cells take u2's values instead
-// (the shell plane k1 is loaded from S1 here; the x-shell and y-shell row
-// values travel with the centre plane in `sh`).
+// (the shell plane k1 is loaded from S1 here, the x-shell values come from
+// the LDS slice, the y-shell rows were loaded one step ahead into `sh`).
 template <int OP, int CX, int RPW, bool SH1>
 __device__ __forceinline__ void sm_finish(SmPlane<CX, RPW> &out, const SmPlane<CX, RPW> &part,
                                           const SmPlane<CX, RPW> &zp, const SmPlane<CX, RPW> &cen,
                                           float kc, bool zshell, const SmTile<RPW> &t,
                                           const SmLane<CX> &l, const SmShell1<CX, RPW> &sh,
-                                          const float *__restrict__ S1, const SmArgs &a, int k)
+                                          const SmXSlice &xsl, int w, const float *__restrict__ S1,
+                                          const SmArgs &a, int k)
 {
     typedef float V4 __attribute__((ext_vector_type(4)));
 #pragma unroll
@@ -375,10 +378,12 @@ __device__ __forceinline__ void sm_finish(SmPlane<CX, RPW> &out, const SmPlane<C
 #pragma unroll
             for (int c = 0; c < CX; ++c) out.v[r][c] = (SH1 && t.ytrue[r]) ? sh.y1[r][c] : cen.v[r][c];
         } else {
+            float2 x1 = make_float2(0.0f, 0.0f);
+            if constexpr (SH1) x1 = xsl.at(1, w * RPW + r, k);
 #pragma unroll
             for (int h = 0; h < CX / 4; ++h) {
-                const float a0 = SH1 ? sh.x1[r].x : cen.v[r][4 * h], b0 = out.v[r][4 * h];
-                const float a1 = SH1 ? sh.x1[r].y : cen.v[r][4 * h + 3], b1 = out.v[r][4 * h + 3];
+                const float a0 = SH1 ? x1.x : cen.v[r][4 * h], b0 = out.v[r][4 * h];
+                const float a1 = SH1 ? x1.y : cen.v[r][4 * h + 3], b1 = out.v[r][4 * h + 3];
                 out.v[r][4 * h] = l.lo[h] ? a0 : b0;
                 out.v[r][4 * h + 3] = l.hi[h] ? a1 : b1;
             }
@@ -441,6 +446,10 @@ __device__ __forceinline__ void sm_pass(const float *__restrict__ X, float *__re
     }
     float4 *reg0 = lds;
     float4 *reg1 = lds + 2 * wy * (CX / 4) * 64;
+    SmXSlice xsl;
+    xsl.v = reinterpret_cast<float2 *>(lds + 4 * wy * (CX / 4) * 64);
+    xsl.rows = wy * RPW;
+    xsl.stride = a.zchunk + 5;
 
     for (int tile = (int)bid; tile < a.ntiles; tile += (int)nblk) {
         const int tyi = tile % a.tiles_y, tzi = tile / a.tiles_y;
@@ -460,33 +469,50 @@ __device__ __forceinline__ void sm_pass(const float *__restrict__ X, float *__re
         }
         t.z0 = tzi * a.zchunk;
         t.z1 = min(t.z0 + a.zchunk, a.nz);
+        if constexpr (DIFF) {
+            // the tile's x-shell values, planes z0-3 .. z1+1 (level 0 reads
+            // z0-2 .. z1+1, level 1 z0-3 .. z1), both buffers: contiguous
+            // runs of the table per row
+            xsl.k0 = t.z0 - 3;
+            const int nk = t.z1 - t.z0 + 5;
+            const int n = 2 * xsl.rows * nk;
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                const int row = i / nk;  // b * rows + frame row
+                const int kk = i - row * nk;
+                const int b = row >= xsl.rows ? 1 : 0;
+                const int j = min(max(fy0 + row - b * xsl.rows, 0), a.ny - 1);
+                const int k = min(max(xsl.k0 + kk, 0), a.nz - 1);
+                xsl.v[row * xsl.stride + kk] =
+                    *reinterpret_cast<const float2 *>(xs + xshell_index(b, k, j, 0, a.ny, a.nz));
+            }
+            __syncthreads();
+        }
 
         // Pipeline state: the prefetched level-0 plane, the previous level-0
         // and level-1 planes, and the running partial sums P1/P2 (updated in
         // place once the finished plane has taken their old values).
-        // (DIFF: the level-0 x-shell values of the prefetched plane ride
-        // along in pfs; sh1 holds the level-1 shell values of the plane the
-        // step finishes at level 1)
+        // (DIFF: sh1 holds the level-1 y-shell rows of the plane the step
+        // finishes at level 1)
         SmPlane<CX, RPW> pf, c0, c1, P1, P2;
-        SmShell0<RPW> pfs;
         SmShell1<CX, RPW> sh1;
 #pragma unroll
         for (int r = 0; r < RPW; ++r)
 #pragma unroll
             for (int c = 0; c < CX; ++c) c0.v[r][c] = c1.v[r][c] = P1.v[r][c] = P2.v[r][c] = 0.0f;
         const int kin_lo = t.z0 - T, kin_hi = t.z1 + T;
-        sm_load<CX, RPW, DIFF>(pf, pfs, X, S0, xs, a, t, l, kin_lo);
-        if constexpr (DIFF) sm_load_sh1<CX, RPW>(sh1, S1, xs, a, t, l, kin_lo - 1);
+        sm_load<CX, RPW, DIFF>(pf, X, S0, a, t, l, kin_lo);
+        if constexpr (DIFF) sm_load_sh1<CX, RPW>(sh1, S1, a, t, l, kin_lo - 1);
         for (int kin = kin_lo; kin < kin_hi; ++kin) {
             SmPlane<CX, RPW> nw0;
-            sm_take<CX, RPW, DIFF>(nw0, pf, pfs, l);
-            if (kin + 1 < kin_hi) sm_load<CX, RPW, DIFF>(pf, pfs, X, S0, xs, a, t, l, kin + 1);
+            sm_take<CX, RPW, DIFF>(nw0, pf, xsl, w, kin, l);
+            if (kin + 1 < kin_hi) sm_load<CX, RPW, DIFF>(pf, X, S0, a, t, l, kin + 1);
             // level 1, plane kin-1
             const int k1 = kin - 1;
             SmPlane<CX, RPW> o1;
-            sm_finish<OP, CX, RPW, DIFF>(o1, P1, nw0, c0, kc, k1 <= 0 || k1 >= a.nz - 1, t, l, sh1, S1, a, k1);
-            // the level-1 shell values the next step needs (plane kin)
-            if constexpr (DIFF) sm_load_sh1<CX, RPW>(sh1, S1, xs, a, t, l, kin);
+            sm_finish<OP, CX, RPW, DIFF>(o1, P1, nw0, c0, kc, k1 <= 0 || k1 >= a.nz - 1, t, l, sh1, xsl, w, S1, a,
+                                         k1);
+            // the level-1 y-shell rows the next step needs (plane kin)
+            if constexpr (DIFF) sm_load_sh1<CX, RPW>(sh1, S1, a, t, l, kin);
             // one sweep: alternate the two regions so that a plane's LDS
             // writes never meet the previous plane's reads
             sm_partial<CX, RPW>(P1, nw0, c0, (T == 1 && (kin & 1)) ? reg1 : reg0, w, wy, lane);
@@ -497,14 +523,14 @@ __device__ __forceinline__ void sm_pass(const float *__restrict__ X, float *__re
                 // level 2, plane kin-2
                 const int k2 = kin - 2;
                 SmPlane<CX, RPW> o2;
-                sm_finish<OP, CX, RPW, false>(o2, P2, o1, c1, kc, k2 <= 0 || k2 >= a.nz - 1, t, l, sh1, S1, a,
-                                              k2);
+                sm_finish<OP, CX, RPW, false>(o2, P2, o1, c1, kc, k2 <= 0 || k2 >= a.nz - 1, t, l, sh1, xsl, w,
+                                              S1, a, k2);
                 sm_partial<CX, RPW>(P2, o1, c1, reg1, w, wy, lane);
                 c1 = o1;
                 sm_store<CX, RPW>(Y, o2, a, t, l, k2);
             }
         }
-        __syncthreads();  // the next tile rewrites both LDS regions
+        __syncthreads();  // the next tile rewrites both LDS regions (and the slice)
     }
 }
 
@@ -596,14 +622,29 @@ static int stream_max_waves(int cx, int rpw)
     return std::min(sm_max_threads<4, 2, false>(), sm_max_threads<4, 2, true>()) / 64;
 }
 
+// LDS of a workgroup: the two exchange regions (2*wy edge rows each, in
+// float4 of CX/4 quads per lane), then the tile's x-shell slice (2 buffers
+// x frame rows x (chunk + 5) planes, float2 each).
+static int stream_lds(int wy, int rpw, int cx, int chunk)
+{
+    return 2 * (2 * wy * (cx / 4) * 64 * 16) + 2 * wy * rpw * (chunk + 5) * 8;
+}
+
+// Deepest z chunk whose slice still fits the LDS.
+static int stream_max_chunk(int wy, int rpw, int cx)
+{
+    return (kMaxLds - stream_lds(wy, rpw, cx, -5)) / (2 * wy * rpw * 8) - 5;
+}
+
 // Tiles of one pass for `wy` waves of `rpw` rows per frame: y frames times
-// z chunks, the chunk count filling the CUs once.
+// z chunks, the chunk count filling the CUs once (chunks no deeper than the
+// LDS slice allows).
 struct SmPlan {
     int wy, core_h, tiles_y, chunk, ntiles;
     double cost;
 };
 
-static SmPlan stream_plan(const Cfg &c, int wy, int rpw, int cus, int chunk_req)
+static SmPlan stream_plan(const Cfg &c, int wy, int rpw, int cx, int cus, int chunk_req)
 {
     SmPlan p;
     p.wy = wy;
@@ -616,6 +657,11 @@ static SmPlan stream_plan(const Cfg &c, int wy, int rpw, int cus, int chunk_req)
         chunk = (c.nz + chunks - 1) / chunks;
         if (chunk < 4) chunk = 4;
         if (chunk > c.nz) chunk = c.nz;
+        const int cmax = stream_max_chunk(wy, rpw, cx);
+        if (chunk > cmax) {
+            const int n = (c.nz + cmax - 1) / cmax;
+            chunk = (c.nz + n - 1) / n;
+        }
     }
     p.chunk = chunk;
     p.ntiles = p.tiles_y * ((c.nz + chunk - 1) / chunk);
@@ -653,19 +699,19 @@ st_status resolve_stream(Cfg &c)
         // block_y = waves per frame (frame of rpw*block_y rows)
         if (c.by < wmin || c.by > wmax)
             return fail(ST_EINVAL, "streaming form: block_y (waves per frame) must be %d..%d", wmin, wmax);
-        best = stream_plan(c, c.by, rpw, cus, chunk_req);
+        best = stream_plan(c, c.by, rpw, cx, cus, chunk_req);
     } else {
         // whole grid in one frame if it fits, else the frame height whose
         // tiles fill the CUs best
         const int wfit = (c.ny + 4 + rpw - 1) / rpw;
-        best = stream_plan(c, wfit < wmax ? (wfit < wmin ? wmin : wfit) : wmax, rpw, cus, chunk_req);
+        best = stream_plan(c, wfit < wmax ? (wfit < wmin ? wmin : wfit) : wmax, rpw, cx, cus, chunk_req);
         // candidate frame heights: down to two thirds of the largest, and
         // not below 8 waves where 8 fit (fewer waves lose the latency
         // hiding; 512^3: 6 waves x 3 chunks ran 21% slower than 8 x 5)
         const int wlo = std::max(std::max(wmin, (wmax * 2) / 3), std::min(wmax, 8));
         if (wfit > wmax)
             for (int wy = wmax - 1; wy >= wlo; --wy) {
-                const SmPlan q = stream_plan(c, wy, rpw, cus, chunk_req);
+                const SmPlan q = stream_plan(c, wy, rpw, cx, cus, chunk_req);
                 if (q.cost < best.cost * 0.97) best = q;  // fewer waves must pay for themselves
             }
         // Fields of 2.5 times the 256 MiB last-level cache and more: the
@@ -686,10 +732,12 @@ st_status resolve_stream(Cfg &c)
                 for (int nch = 1; nch * tiles_y <= cus; ++nch) {
                     const int chunk = (c.nz + nch - 1) / nch;
                     const int ntiles = tiles_y * ((c.nz + chunk - 1) / chunk);
-                    if (chunk < 4 || ntiles * 100 < cus * 75 || ntiles * 100 > cus * 92) continue;
+                    if (chunk < 4 || chunk > stream_max_chunk(wy, rpw, cx) || ntiles * 100 < cus * 75 ||
+                        ntiles * 100 > cus * 92)
+                        continue;
                     const double amp = (chunk + 4.0) / chunk;
                     if (pick.wy == 0 || amp < pick_amp - 1e-9 || (amp < pick_amp + 1e-9 && ntiles > pick.ntiles)) {
-                        pick = stream_plan(c, wy, rpw, cus, chunk);
+                        pick = stream_plan(c, wy, rpw, cx, cus, chunk);
                         pick_amp = amp;
                     }
                 }
@@ -703,7 +751,10 @@ st_status resolve_stream(Cfg &c)
     c.wx = 1;
     c.wy = best.wy;
     c.tsteps = 2;
-    c.lds = 2 * (2 * best.wy * (cx / 4) * 64 * 16);
+    c.lds = stream_lds(best.wy, rpw, cx, best.chunk);
+    if (c.lds > kMaxLds)
+        return fail(ST_ELDS, "streaming form: a z chunk of %d planes needs %d bytes of LDS (at most %d "
+                             "planes with this frame)", best.chunk, c.lds, stream_max_chunk(best.wy, rpw, cx));
     c.bx = c.nx;
     c.by = best.core_h;
     c.zchunk = best.chunk;
