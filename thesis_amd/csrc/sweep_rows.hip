@@ -491,6 +491,25 @@ constexpr int rowc_max_threads()
     return w <= 128 ? 1024 : w <= 150 ? 768 : w <= 200 ? 512 : 0;
 }
 
+// N consecutive 32-bit words from a 4-byte aligned address, in vectors of
+// 4, 2 or 1 words.
+template <int N>
+__device__ __forceinline__ void load_words(const uint32_t *__restrict__ p, uint32_t (&d)[N])
+{
+    constexpr int V = N % 4 == 0 ? 4 : N % 2 == 0 ? 2 : 1;
+#pragma unroll
+    for (int v = 0; v < N / V; ++v) {
+        if constexpr (V == 1) {
+            d[v] = p[v];
+        } else {
+            typedef typename VecN<uint32_t, V>::type W;
+            const W q = *reinterpret_cast<const W *>(p + v * V);
+#pragma unroll
+            for (int j = 0; j < V; ++j) d[v * V + j] = q[j];
+        }
+    }
+}
+
 template <int OP, int R, int CX, typename T, int MODE, bool F16>
 __global__ __launch_bounds__((rowc_max_threads<R, CX, T>())) void k_rowc(
     const void *__restrict__ in0, void *__restrict__ out0, const uint16_t *__restrict__ aux, RowsArgs a, T kc)
@@ -499,6 +518,9 @@ __global__ __launch_bounds__((rowc_max_threads<R, CX, T>())) void k_rowc(
     static_assert(MODE == RM_SWEEP || sizeof(T) == 4, "the 16-bit modes compute in FP32");
     // storage type of the input field
     typedef typename std::conditional<MODE == RM_CORR, uint16_t, T>::type S;
+    // prefetch storage: 16-bit inputs as packed words
+    typedef typename std::conditional<MODE == RM_CORR, uint32_t, T>::type P;
+    constexpr int NP = MODE == RM_CORR ? CX / 2 : CX;
     constexpr int VS = rowc_vec<CX, (int)sizeof(S)>();  // global loads
     constexpr int VT = rowc_vec<CX, (int)sizeof(T)>();  // LDS and sweep stores
     constexpr int VH = rowc_vec<CX, 2>();                // 16-bit stores and defect loads
@@ -540,21 +562,41 @@ __global__ __launch_bounds__((rowc_max_threads<R, CX, T>())) void k_rowc(
         const int z0 = tzi * a.zchunk, z1 = min(z0 + a.zchunk, a.nz);
         const int k_out_lo = max(z0, R), k_out_hi = min(z1, a.nz - R);
 
-        auto load_row = [&](int k, int off, T (&d)[CX]) {
+        // Prefetched rows are held as loaded: 16-bit values stay packed two
+        // per word and are widened when the plane is consumed (widened at
+        // the load, every conversion waited for the load in flight, which
+        // made each step wait for its own prefetch)
+        auto load_row = [&](int k, int off, P (&d)[NP]) {
             k = min(max(k, 0), a.nz - 1);
-            const S *p = static_cast<const S *>(in_) + (long long)k * a.sz + off + xl;
+            if constexpr (MODE == RM_CORR) {
+                const uint32_t *p = reinterpret_cast<const uint32_t *>(static_cast<const uint16_t *>(in_) +
+                                                                       (long long)k * a.sz + off + xl);
+                load_words<NP>(p, d);
+            } else {
+                const S *p = static_cast<const S *>(in_) + (long long)k * a.sz + off + xl;
 #pragma unroll
-            for (int v = 0; v < CX / VS; ++v) {
-                const SV q = *reinterpret_cast<const SV *>(p + v * VS);
+                for (int v = 0; v < CX / VS; ++v) {
+                    const SV q = *reinterpret_cast<const SV *>(p + v * VS);
 #pragma unroll
-                for (int j = 0; j < VS; ++j) {
-                    if constexpr (MODE == RM_CORR) d[v * VS + j] = wid16<F16>(q[j]);
-                    else d[v * VS + j] = q[j];
+                    for (int j = 0; j < VS; ++j) d[v * VS + j] = q[j];
                 }
             }
         };
+        auto widen_row = [&](const P (&src)[NP], T (&d)[CX]) {
+            if constexpr (MODE == RM_CORR) {
+#pragma unroll
+                for (int v = 0; v < NP; ++v) {
+                    d[2 * v] = wid16<F16>((uint16_t)(src[v] & 0xffffu));
+                    d[2 * v + 1] = wid16<F16>((uint16_t)(src[v] >> 16));
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < CX; ++c) d[c] = src[c];
+            }
+        };
 
-        T ring[R][CX], part[R][CX], pf[CX], ph[CX];
+        T ring[R][CX], part[R][CX];
+        P pf[NP], ph[NP];
 #pragma unroll
         for (int i = 0; i < R; ++i)
 #pragma unroll
@@ -563,21 +605,16 @@ __global__ __launch_bounds__((rowc_max_threads<R, CX, T>())) void k_rowc(
         load_row(kin_lo, own_off, pf);
         if (has_halo) load_row(kin_lo, halo_off, ph);
         // correction sweep: the defect of the cells a step completes (plane
-        // m-R), loaded in the step after the plane prefetch
-        uint16_t dqn[CX];
+        // m-R), loaded one step ahead, packed like the prefetched rows
+        uint32_t dqn[CX / 2];
         auto load_defect = [&](int k) {
             if constexpr (MODE == RM_CORR) {
-                if (store_row && k >= k_out_lo && k < k_out_hi) {
-                    const uint16_t *p = aux + (long long)k * a.sz + own_off + xl;
-#pragma unroll
-                    for (int v = 0; v < CX / VH; ++v) {
-                        const HV q = *reinterpret_cast<const HV *>(p + v * VH);
-#pragma unroll
-                        for (int j = 0; j < VH; ++j) dqn[v * VH + j] = q[j];
-                    }
-                }
+                if (store_row && k >= k_out_lo && k < k_out_hi)
+                    load_words<CX / 2>(reinterpret_cast<const uint32_t *>(aux + (long long)k * a.sz + own_off + xl),
+                                       dqn);
             }
         };
+        load_defect(kin_lo - R);
 
         for (int m0 = kin_lo; m0 < kin_hi; m0 += R) {
 #pragma unroll
@@ -586,8 +623,7 @@ __global__ __launch_bounds__((rowc_max_threads<R, CX, T>())) void k_rowc(
             if (m >= kin_hi) break;
             __builtin_amdgcn_sched_barrier(0);
             T nw[CX];
-#pragma unroll
-            for (int c = 0; c < CX; ++c) nw[c] = pf[c];
+            widen_row(pf, nw);
             T *buf = lds + (size_t)((m & 1) * lrows) * RS;
 #pragma unroll
             for (int v = 0; v < CX / VT; ++v) {
@@ -597,25 +633,27 @@ __global__ __launch_bounds__((rowc_max_threads<R, CX, T>())) void k_rowc(
                 *reinterpret_cast<TV *>(buf + (R + w) * RS + xo + v * VT) = q;
             }
             if (has_halo) {
+                T hv[CX];
+                widen_row(ph, hv);
 #pragma unroll
                 for (int v = 0; v < CX / VT; ++v) {
                     TV q;
 #pragma unroll
-                    for (int j = 0; j < VT; ++j) q[j] = ph[v * VT + j];
+                    for (int j = 0; j < VT; ++j) q[j] = hv[v * VT + j];
                     *reinterpret_cast<TV *>(buf + hrow * RS + xo + v * VT) = q;
                 }
             }
             const int k = m - R;
             const bool out_plane = store_row && k >= k_out_lo && k < k_out_hi;
+            uint32_t dq[CX / 2];
+            if constexpr (MODE == RM_CORR) {
+#pragma unroll
+                for (int v = 0; v < CX / 2; ++v) dq[v] = dqn[v];
+            }
             if (m + 1 < kin_hi) {
                 load_row(m + 1, own_off, pf);
                 if (has_halo) load_row(m + 1, halo_off, ph);
-            }
-            uint16_t dq[CX];
-            load_defect(k);
-            if constexpr (MODE == RM_CORR) {
-#pragma unroll
-                for (int c = 0; c < CX; ++c) dq[c] = dqn[c];
+                load_defect(k + 1);
             }
             __syncthreads();
 
@@ -700,7 +738,10 @@ __global__ __launch_bounds__((rowc_max_threads<R, CX, T>())) void k_rowc(
                     for (int c = 0; c < CX; ++c) {
                         const float sw = part[rp][c] * kc;
                         uint16_t r16;
-                        if constexpr (MODE == RM_CORR) r16 = rnd16<F16>(wid16<F16>(dq[c]) + sw);
+                        if constexpr (MODE == RM_CORR) {
+                            const uint16_t dc = (uint16_t)(c & 1 ? dq[c / 2] >> 16 : dq[c / 2] & 0xffffu);
+                            r16 = rnd16<F16>(wid16<F16>(dc) + sw);
+                        }
                         else r16 = rnd16<F16>(sw - ring[rp][c]);
                         res[c] = whole || interior1(xo + c, a.nx, R) ? r16 : (uint16_t)0;
                     }
