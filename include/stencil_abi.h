@@ -223,8 +223,9 @@ st_status stencil_residual_norm(const st_params *p, const void *d_u, double *nor
  * p->restart sweeps the correction is folded into x in FP32 and the defect
  * d = G(x) - x is recomputed in FP32.  Each correction sweep is
  *     e' = round16(widen(d) + weight * sum of widen(e) neighbours)
- * d_x (FP32, in/out), d_e1, d_e2, d_d (16-bit, nx*ny*nz each, scratch) and
- * d_u (FP32 out, the final x + e).  Bit-exact with oracle/stencil_oracle.c. */
+ * d_x (FP32 in: the initial field; scratch afterwards), d_e1, d_e2, d_d
+ * (16-bit, nx*ny*nz each, scratch) and d_u (FP32 out, the final x + e; also
+ * scratch during the run).  Bit-exact with oracle/stencil_oracle.c. */
 st_status stencil_dispatch_mixed(const st_params *p, void *d_x, void *d_e1, void *d_e2,
                                  void *d_d, void *d_u, void *stream, float *elapsed_ms);
 

@@ -124,6 +124,16 @@ __global__ __launch_bounds__(256) void k_mx_final(const float *__restrict__ x,
         u[i] = x[i] + wid16<F16>(e[i]);
 }
 
+// u += widen(e) (the final fold when the current field is already in u).
+template <bool F16>
+__global__ __launch_bounds__(256) void k_mx_final_inplace(float *__restrict__ u, const uint16_t *__restrict__ e,
+                                                          long long n)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += stride)
+        u[i] = u[i] + wid16<F16>(e[i]);
+}
+
 template <bool F16>
 static st_status mixed_impl(const Cfg &c, float *x, uint16_t *e1, uint16_t *e2, uint16_t *d,
                             float *u, hipStream_t s)
@@ -148,6 +158,34 @@ static st_status mixed_impl(const Cfg &c, float *x, uint16_t *e1, uint16_t *e2, 
     // loads, same operation order); the rest use the per-cell kernels
     const bool rows = c.dims == 3 && rows_mixed_eligible(c);
     st_status st = ST_OK;
+    if (rows && c.iters > 0 && rows_mixed_fused_ok(c)) {
+        // The same arithmetic with two passes fewer per restart: the defect
+        // sweep also writes the first correction sweep of its cycle
+        // (round16(widen(d) + w*0), which is what a sweep from a zero
+        // correction computes), and the fold x += e of a restart is formed
+        // on the fly by the next defect sweep, which writes x' to the other
+        // FP32 buffer (u, given x's shell first) instead of updating x in
+        // place.  Sweep it (1-based) is the first of its cycle when
+        // (it - 1) % restart == 0.
+        if (hipMemcpyAsync(u, x, n * sizeof(float), hipMemcpyDeviceToDevice, s) != hipSuccess)
+            return hip_check(hipGetLastError(), "mixed: copy of x");
+        float *xa = x, *xb = u;
+        uint16_t *ea = e1, *eb = e2;
+        st = launch_rows_defect(c, F16, xa, nullptr, nullptr, d, ea, s);
+        for (int it = 2; it <= c.iters && st == ST_OK; ++it) {
+            if ((it - 1) % c.restart == 0) {
+                st = launch_rows_defect(c, F16, xa, ea, xb, d, eb, s);
+                float *t = xa; xa = xb; xb = t;
+            } else {
+                st = launch_rows_mixed(c, 0, F16, ea, eb, d, s);
+            }
+            uint16_t *t = ea; ea = eb; eb = t;
+        }
+        if (st != ST_OK) return st;
+        if (xa == u) hipLaunchKernelGGL(k_mx_final_inplace<F16>, blocks(n), dim3(256), 0, s, u, ea, n);
+        else hipLaunchKernelGGL(k_mx_final<F16>, blocks(n), dim3(256), 0, s, xa, ea, u, n);
+        return hip_check(hipGetLastError(), "mixed launch");
+    }
     auto defect = [&]() {
         if (rows) st = launch_rows_mixed(c, 1, F16, x, d, nullptr, s);
         else hipLaunchKernelGGL(k_mx_defect<F16>, blocks(ni), dim3(256), 0, s, x, d, a);

@@ -78,6 +78,16 @@ bool rows_mixed_eligible(const Cfg &c);
 st_status rows_shell_check(const Cfg &c, const void *u1, const void *u2, hipStream_t s, const int **flag);
 st_status launch_rows_mixed(const Cfg &c, int defect, bool f16, const void *in, void *out, const uint16_t *d,
                             hipStream_t s);
+// The consecutive row layout serves the correction, defect and fused fold
+// modes on the geometry of `c`.
+bool rows_mixed_fused_ok(const Cfg &c);
+// Defect sweep d = round16(w*sum(x') - x') that also writes e_first =
+// round16(widen(d) + 0), the first correction sweep from a zero correction
+// (shell zero).  With e_old, x' = x + widen(e_old) is formed on the fly and
+// its interior rows stored to x_out (a buffer other than x, holding x's
+// shell); without, x' = x and x_out is unused.
+st_status launch_rows_defect(const Cfg &c, bool f16, const float *x, const uint16_t *e_old, float *x_out,
+                             uint16_t *d, uint16_t *e_first, hipStream_t s);
 
 // Streaming temporal blocking, whole rows per wave (sweep_stream.hip).
 bool stream_eligible(const Cfg &c);

@@ -56,6 +56,11 @@ struct RowsArgs {
     int sweeps;     // sweeps of this launch (> 1 only for cooperative launches)
     const int *eq;  // sweep mode: 1 when both buffers' shells are equal
                     // (edge quads then store whole, shell cells unchanged)
+    // defect and fold modes of the consecutive layout (null: not written):
+    // eout receives round16(widen(d) + 0), the first correction sweep from
+    // a zero correction; xout the folded field x' (fold mode)
+    uint16_t *eout;
+    float *xout;
 };
 
 template <typename T> struct Quad;
@@ -95,7 +100,10 @@ __device__ __forceinline__ T dpp_next(T v)  // lane i <- lane i+1 (rotating)
 // upper bound for the phase-unrolled loop; shapes above 250 are not built),
 // and the workgroup size whose register budget the kernel fits: 1024
 // threads leave 128 VGPRs, 768 leave 168, 512 leave 256.
-enum : int { RM_SWEEP = 0, RM_CORR = 1, RM_DEFECT = 2 };
+// RM_FOLD (consecutive layout only): x' = x + widen(e) on the fly, x'
+// stored, then the defect of x' (the fold of a restart fused into its
+// defect sweep)
+enum : int { RM_SWEEP = 0, RM_CORR = 1, RM_DEFECT = 2, RM_FOLD = 3 };
 
 template <int R, int CX, typename T, int MODE = RM_SWEEP, int HS = 1>
 constexpr int rows_vgprs()
@@ -516,6 +524,8 @@ __global__ __launch_bounds__((rowc_max_threads<R, CX, T>())) void k_rowc(
 {
     static_assert(R >= 1 && R <= CX, "x neighbours come from the adjacent lane");
     static_assert(MODE == RM_SWEEP || sizeof(T) == 4, "the 16-bit modes compute in FP32");
+    constexpr bool FOLD = MODE == RM_FOLD;
+    constexpr int NE = FOLD ? CX / 2 : 1;  // packed correction words per row (fold)
     // storage type of the input field
     typedef typename std::conditional<MODE == RM_CORR, uint16_t, T>::type S;
     // prefetch storage: 16-bit inputs as packed words
@@ -582,6 +592,20 @@ __global__ __launch_bounds__((rowc_max_threads<R, CX, T>())) void k_rowc(
                 }
             }
         };
+        auto load_e = [&](int k, int off, uint32_t (&d)[NE]) {
+            if constexpr (FOLD) {
+                k = min(max(k, 0), a.nz - 1);
+                load_words<NE>(reinterpret_cast<const uint32_t *>(aux + (long long)k * a.sz + off + xl), d);
+            }
+        };
+        // x + widen(e), in the order of the separate fold (fold mode)
+        auto fold_row = [&](T (&d)[CX], const uint32_t (&e)[NE]) {
+            if constexpr (FOLD) {
+#pragma unroll
+                for (int c = 0; c < CX; ++c)
+                    d[c] = d[c] + wid16<F16>((uint16_t)(c & 1 ? e[c / 2] >> 16 : e[c / 2] & 0xffffu));
+            }
+        };
         auto widen_row = [&](const P (&src)[NP], T (&d)[CX]) {
             if constexpr (MODE == RM_CORR) {
 #pragma unroll
@@ -597,13 +621,18 @@ __global__ __launch_bounds__((rowc_max_threads<R, CX, T>())) void k_rowc(
 
         T ring[R][CX], part[R][CX];
         P pf[NP], ph[NP];
+        uint32_t pe[NE], phe[NE];
 #pragma unroll
         for (int i = 0; i < R; ++i)
 #pragma unroll
             for (int c = 0; c < CX; ++c) ring[i][c] = part[i][c] = (T)0;
         const int kin_lo = z0 - R, kin_hi = z1 + R;
         load_row(kin_lo, own_off, pf);
-        if (has_halo) load_row(kin_lo, halo_off, ph);
+        load_e(kin_lo, own_off, pe);
+        if (has_halo) {
+            load_row(kin_lo, halo_off, ph);
+            load_e(kin_lo, halo_off, phe);
+        }
         // correction sweep: the defect of the cells a step completes (plane
         // m-R), loaded one step ahead, packed like the prefetched rows
         uint32_t dqn[CX / 2];
@@ -624,6 +653,7 @@ __global__ __launch_bounds__((rowc_max_threads<R, CX, T>())) void k_rowc(
             __builtin_amdgcn_sched_barrier(0);
             T nw[CX];
             widen_row(pf, nw);
+            fold_row(nw, pe);
             T *buf = lds + (size_t)((m & 1) * lrows) * RS;
 #pragma unroll
             for (int v = 0; v < CX / VT; ++v) {
@@ -635,6 +665,7 @@ __global__ __launch_bounds__((rowc_max_threads<R, CX, T>())) void k_rowc(
             if (has_halo) {
                 T hv[CX];
                 widen_row(ph, hv);
+                fold_row(hv, phe);
 #pragma unroll
                 for (int v = 0; v < CX / VT; ++v) {
                     TV q;
@@ -652,7 +683,11 @@ __global__ __launch_bounds__((rowc_max_threads<R, CX, T>())) void k_rowc(
             }
             if (m + 1 < kin_hi) {
                 load_row(m + 1, own_off, pf);
-                if (has_halo) load_row(m + 1, halo_off, ph);
+                load_e(m + 1, own_off, pe);
+                if (has_halo) {
+                    load_row(m + 1, halo_off, ph);
+                    load_e(m + 1, halo_off, phe);
+                }
                 load_defect(k + 1);
             }
             __syncthreads();
@@ -745,13 +780,38 @@ __global__ __launch_bounds__((rowc_max_threads<R, CX, T>())) void k_rowc(
                         else r16 = rnd16<F16>(sw - ring[rp][c]);
                         res[c] = whole || interior1(xo + c, a.nx, R) ? r16 : (uint16_t)0;
                     }
-                    uint16_t *row = static_cast<uint16_t *>(out_) + (long long)k * a.sz + (long long)gy * a.nx + xo;
+                    const long long ro = (long long)k * a.sz + (long long)gy * a.nx + xo;
+                    auto store16 = [&](uint16_t *row, const uint16_t (&v16)[CX]) {
 #pragma unroll
-                    for (int v = 0; v < CX / VH; ++v) {
-                        HV q;
+                        for (int v = 0; v < CX / VH; ++v) {
+                            HV q;
 #pragma unroll
-                        for (int j = 0; j < VH; ++j) q[j] = res[v * VH + j];
-                        *reinterpret_cast<HV *>(row + v * VH) = q;
+                            for (int j = 0; j < VH; ++j) q[j] = v16[v * VH + j];
+                            *reinterpret_cast<HV *>(row + v * VH) = q;
+                        }
+                    };
+                    store16(static_cast<uint16_t *>(out_) + ro, res);
+                    if constexpr (MODE != RM_CORR) {
+                        if (a.eout) {
+                            // the first correction sweep from a zero
+                            // correction: round16(widen(d) + w*0), shell 0
+                            uint16_t e1[CX];
+#pragma unroll
+                            for (int c = 0; c < CX; ++c)
+                                e1[c] = whole || interior1(xo + c, a.nx, R) ? rnd16<F16>(wid16<F16>(res[c]) + 0.0f)
+                                                                           : (uint16_t)0;
+                            store16(a.eout + ro, e1);
+                        }
+                    }
+                    if constexpr (FOLD) {
+                        // x' of the row, whole (shell cells: x + widen(0))
+#pragma unroll
+                        for (int v = 0; v < CX / VT; ++v) {
+                            TV q;
+#pragma unroll
+                            for (int j = 0; j < VT; ++j) q[j] = ring[rp][v * VT + j];
+                            *reinterpret_cast<TV *>(a.xout + ro + v * VT) = q;
+                        }
                     }
                 }
             }
@@ -871,6 +931,9 @@ static const void *rowc_fn_mode(const Cfg &c, int cx, int mode, bool f16, int *t
     if (mode == RM_DEFECT)
         return f16 ? rowc_fn_d<float, RM_DEFECT, true>(c.op, c.r, cx, threads)
                    : rowc_fn_d<float, RM_DEFECT, false>(c.op, c.r, cx, threads);
+    if (mode == RM_FOLD)
+        return f16 ? rowc_fn_d<float, RM_FOLD, true>(c.op, c.r, cx, threads)
+                   : rowc_fn_d<float, RM_FOLD, false>(c.op, c.r, cx, threads);
     if (c.dtype == ST_F64) return rowc_fn_d<double, RM_SWEEP, false>(c.op, c.r, cx, threads);
     return rowc_fn_d<float, RM_SWEEP, false>(c.op, c.r, cx, threads);
 }
@@ -943,6 +1006,8 @@ static st_status rows_resolve_mode(Cfg &c, int mode, bool f16)
         fn = rowc_fn_mode(c, cx, mode, f16, &threads);
         if (fn && threads / 64 >= 2 * c.r) c.rowc = 1;
     }
+    if (!c.rowc && mode == RM_FOLD)
+        return fail(ST_EUNSUPPORTED, "the fused fold and defect sweep needs rows of 256, 384 or 512 columns");
     if (!c.rowc && (c.dims != 3 || !rows_layout(c, mode, f16, &cx, &wx, &threads, &fn, &hs)))
         return fail(ST_EUNSUPPORTED, "row-streaming form needs a 3D F32/F64 grid, radius 1..4 and nx a "
                                      "multiple of 4 whose rows fit a workgroup with its halo strips");
@@ -1013,9 +1078,11 @@ st_status resolve_rows(Cfg &c) { return rows_resolve_mode(c, RM_SWEEP, false); }
 // after each (the grid is capped at the co-resident workgroups; the tile loop
 // strides over the rest)
 static st_status rows_launch(const Cfg &c, const void *fn, const void *in, void *out, const uint16_t *aux,
-                             hipStream_t s, int sweeps = 1)
+                             hipStream_t s, int sweeps = 1, uint16_t *eout = nullptr, float *xout = nullptr)
 {
     RowsArgs a;
+    a.eout = eout;
+    a.xout = xout;
     a.nx = c.nx; a.ny = c.ny; a.nz = c.nz;
     a.sz = (long long)c.nx * c.ny;
     a.wx = c.wx;
@@ -1161,6 +1228,36 @@ st_status rows_shell_check(const Cfg &c, const void *u1, const void *u2, hipStre
 
 // The 16-bit modes resolve their own layout (their register need differs
 // from the sweep's by the defect row).
+bool rows_mixed_fused_ok(const Cfg &c)
+{
+    if (c.dims != 3) return false;
+    for (int mode : {RM_CORR, RM_DEFECT, RM_FOLD}) {
+        Cfg f = c;
+        f.dtype = ST_F32;
+        f.op = OP_JACOBI;
+        f.heuristic = 1;
+        if (rows_resolve_mode(f, mode, c.dtype == ST_F16) != ST_OK || !f.rowc) return false;
+    }
+    return true;
+}
+
+st_status launch_rows_defect(const Cfg &c, bool f16, const float *x, const uint16_t *e_old, float *x_out,
+                             uint16_t *d, uint16_t *e_first, hipStream_t s)
+{
+    Cfg rc = c;
+    rc.dtype = ST_F32;
+    rc.op = OP_JACOBI;
+    rc.heuristic = 1;
+    const int mode = e_old ? RM_FOLD : RM_DEFECT;
+    st_status st = rows_resolve_mode(rc, mode, f16);
+    if (st != ST_OK) return st;
+    if (!rc.rowc) return fail(ST_EUNSUPPORTED, "fused defect sweep needs the consecutive row layout");
+    int threads = 0;
+    const void *fn = rows_kernel(rc, mode, f16, &threads);
+    if (!fn || 64 * rc.wx * rc.wy > threads) return fail(ST_EUNSUPPORTED, "row-streaming defect kernel missing");
+    return rows_launch(rc, fn, x, d, e_old, s, 1, e_first, x_out);
+}
+
 bool rows_mixed_eligible(const Cfg &c)
 {
     Cfg f = c;
