@@ -96,14 +96,15 @@ __device__ __forceinline__ T dpp_next(T v)  // lane i <- lane i+1 (rotating)
     }
 }
 
+// Kernel modes: a sweep, the two 16-bit modes, and RM_FOLD (consecutive
+// layout only): x' = x + widen(e) on the fly, x' stored, then the defect of
+// x' (the fold of a restart fused into its defect sweep).
+enum : int { RM_SWEEP = 0, RM_CORR = 1, RM_DEFECT = 2, RM_FOLD = 3 };
+
 // VGPRs a lane needs with the ring and partials shifted every plane (an
 // upper bound for the phase-unrolled loop; shapes above 250 are not built),
 // and the workgroup size whose register budget the kernel fits: 1024
 // threads leave 128 VGPRs, 768 leave 168, 512 leave 256.
-// RM_FOLD (consecutive layout only): x' = x + widen(e) on the fly, x'
-// stored, then the defect of x' (the fold of a restart fused into its
-// defect sweep)
-enum : int { RM_SWEEP = 0, RM_CORR = 1, RM_DEFECT = 2, RM_FOLD = 3 };
 
 template <int R, int CX, typename T, int MODE = RM_SWEEP, int HS = 1>
 constexpr int rows_vgprs()
@@ -142,6 +143,9 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
     constexpr int H = CX / 4;
     typedef typename Quad<T>::type QV;
     typedef uint16_t U4 __attribute__((ext_vector_type(4)));
+    // prefetch storage: 16-bit inputs as packed words
+    typedef typename std::conditional<MODE == RM_CORR, uint32_t, T>::type P;
+    constexpr int NP = MODE == RM_CORR ? CX / 2 : CX;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     QV *lds = reinterpret_cast<QV *>(lds_raw);
     const int lane = threadIdx.x & 63;
@@ -204,15 +208,18 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
         }
         const int z0 = tzi * a.zchunk, z1 = min(z0 + a.zchunk, a.nz);
 
-        auto load_row = [&](int k, int off, const int (&cols)[H], T (&d)[CX]) {
+        // prefetched rows are held as loaded (16-bit quads as two packed
+        // words, widened when the plane is consumed: widening at the load
+        // made each step wait for its own prefetch)
+        auto load_row = [&](int k, int off, const int (&cols)[H], P (&d)[NP]) {
             k = min(max(k, 0), a.nz - 1);
             if constexpr (MODE == RM_CORR) {
                 const uint16_t *p = static_cast<const uint16_t *>(in_) + (long long)k * a.sz + off;
 #pragma unroll
                 for (int h = 0; h < H; ++h) {
-                    const U4 q = *reinterpret_cast<const U4 *>(p + cols[h]);
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) d[4 * h + j] = wid16<F16>(q[j]);
+                    const uint2 q = *reinterpret_cast<const uint2 *>(p + cols[h]);
+                    d[2 * h] = q.x;
+                    d[2 * h + 1] = q.y;
                 }
             } else {
                 const T *p = static_cast<const T *>(in_) + (long long)k * a.sz + off;
@@ -226,7 +233,21 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
         };
         const int k_out_lo = max(z0, R), k_out_hi = min(z1, a.nz - R);
 
-        T ring[R][CX], part[R][CX], pf[CX], ph[HS][CX];
+        auto widen_row = [&](const P (&src)[NP], T (&d)[CX]) {
+            if constexpr (MODE == RM_CORR) {
+#pragma unroll
+                for (int v = 0; v < NP; ++v) {
+                    d[2 * v] = wid16<F16>((uint16_t)(src[v] & 0xffffu));
+                    d[2 * v + 1] = wid16<F16>((uint16_t)(src[v] >> 16));
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < CX; ++c) d[c] = src[c];
+            }
+        };
+
+        T ring[R][CX], part[R][CX];
+        P pf[NP], ph[HS][NP];
 #pragma unroll
         for (int i = 0; i < R; ++i)
 #pragma unroll
@@ -237,18 +258,18 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
         for (int i = 0; i < HS; ++i)
             if (has_halo[i]) load_row(kin_lo, halo_off[i], hxo[i], ph[i]);
         // correction sweep: the defect at the cells a step completes (plane
-        // m-R), loaded in the step after the plane prefetch (loading it one
-        // step ahead measured slower: more registers, fewer waves)
-        U4 dqn[H];
+        // m-R), loaded one step ahead, packed like the prefetched rows
+        uint2 dqn[H];
         auto load_defect = [&](int k) {
             if constexpr (MODE == RM_CORR) {
                 if (store_row && k >= k_out_lo && k < k_out_hi) {
                     const uint16_t *p = aux + (long long)k * a.sz + own_off;
 #pragma unroll
-                    for (int h = 0; h < H; ++h) dqn[h] = *reinterpret_cast<const U4 *>(p + xo[h]);
+                    for (int h = 0; h < H; ++h) dqn[h] = *reinterpret_cast<const uint2 *>(p + xo[h]);
                 }
             }
         };
+        load_defect(kin_lo - R);
 
         // planes in phases of R: the ring and the partial sums rotate
         // through their R slots (logical entry i of phase rp in slot
@@ -262,8 +283,7 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
             if (m >= kin_hi) break;
             __builtin_amdgcn_sched_barrier(0);
             T nw[CX];
-#pragma unroll
-            for (int c = 0; c < CX; ++c) nw[c] = pf[c];
+            widen_row(pf, nw);
             QV *buf = lds + (size_t)((m & 1) * lrows) * QW * 64;
 #pragma unroll
             for (int h = 0; h < H; ++h) {
@@ -275,27 +295,29 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
 #pragma unroll
             for (int i = 0; i < HS; ++i) {
                 if (!has_halo[i]) continue;
+                T hv[CX];
+                widen_row(ph[i], hv);
 #pragma unroll
                 for (int h = 0; h < H; ++h) {
                     QV q;
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) q[j] = ph[i][4 * h + j];
+                    for (int j = 0; j < 4; ++j) q[j] = hv[4 * h + j];
                     buf[(hrow[i] * QW + hp[i] * H + h) * 64 + lane] = q;
                 }
             }
             const int k = m - R;
             const bool out_plane = store_row && k >= k_out_lo && k < k_out_hi;
+            uint2 dq[H];
+            if constexpr (MODE == RM_CORR) {
+#pragma unroll
+                for (int h = 0; h < H; ++h) dq[h] = dqn[h];
+            }
             if (m + 1 < kin_hi) {
                 load_row(m + 1, own_off, xo, pf);
 #pragma unroll
                 for (int i = 0; i < HS; ++i)
                     if (has_halo[i]) load_row(m + 1, halo_off[i], hxo[i], ph[i]);
-            }
-            U4 dq[H];
-            load_defect(k);
-            if constexpr (MODE == RM_CORR) {
-#pragma unroll
-                for (int h = 0; h < H; ++h) dq[h] = dqn[h];
+                load_defect(k + 1);
             }
             __syncthreads();
 
@@ -430,7 +452,11 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
                             const float sw = part[rp][4 * h + j] * kc;
-                            if constexpr (MODE == RM_CORR) res[h][j] = rnd16<F16>(wid16<F16>(dq[h][j]) + sw);
+                            if constexpr (MODE == RM_CORR) {
+                                const uint32_t wd = j < 2 ? dq[h].x : dq[h].y;
+                                const uint16_t dc = (uint16_t)(j & 1 ? wd >> 16 : wd & 0xffffu);
+                                res[h][j] = rnd16<F16>(wid16<F16>(dc) + sw);
+                            }
                             else res[h][j] = rnd16<F16>(sw - ring[rp][4 * h + j]);
                         }
                     uint16_t *row = static_cast<uint16_t *>(out_) + (long long)k * a.sz + (long long)gy * a.nx;
