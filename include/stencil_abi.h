@@ -235,7 +235,10 @@ st_status stencil_dispatch_mixed(const st_params *p, void *d_x, void *d_e1, void
 
 /* One grid of a batch; the per-problem contract is stencil_dispatch's:
  * `iterations` sweeps ping-ponging between u1 and u2, result in u1 for an
- * even count, boundary shell of both buffers read-only. */
+ * even count, boundary shell of both buffers read-only.  In ST_BATCH_GRAPH a
+ * problem whose two buffers hold the same shell takes passes of two fused
+ * sweeps, which leave an intermediate state (not the previous sweep) in the
+ * other buffer's interior, as stencil_dispatch's streaming form does. */
 typedef struct st_problem {
     int nx, ny, nz;
     int radius;
@@ -259,7 +262,7 @@ typedef struct st_tile_rule {
 } st_tile_rule;
 
 typedef enum st_batch_mode {
-    ST_BATCH_GRAPH = 0,       /* one hipGraph replaying a batched launch per sweep */
+    ST_BATCH_GRAPH = 0,       /* one hipGraph replaying a batched launch per pass  */
     ST_BATCH_PERSISTENT = 1,  /* one cooperative launch, grid barrier per sweep    */
     ST_BATCH_SEQUENTIAL = 2   /* stencil_dispatch per problem (baseline)           */
 } st_batch_mode;

@@ -40,10 +40,16 @@ struct BatchProblem {
     float kc;       // THESIS divisor or JACOBI weight
     int op;
     int iters;
-    int ty;         // rows per wave (1, 2, 4)
+    int ty;         // rows per wave (1, 2, 4); plane form: frame rows
     int tiles_x, tiles_y;
     int zchunk;
     int item0;      // first item of this problem in the flat list
+    // plane form, two fused sweeps per pass (equal shells): frame rows, y
+    // tiles, first item in the fused list; n2 fused passes (even) come
+    // first, then iters - 2*n2 single sweeps (0: single sweeps only)
+    int tyf, tiles_yf, item0f;
+    int zchunkf;    // planes per fused tile
+    int n2;
     float *u1, *u2;
 };
 
@@ -286,6 +292,192 @@ __device__ __forceinline__ void bp_tile(const BatchProblem &q, int local, int sw
     __syncthreads();  // the next tile rewrites the LDS planes
 }
 
+// ---------------------------------------------------------------- plane form, two fused sweeps
+//
+// Problems whose two buffers hold the same shell (a shell cell has the same
+// value at every time level) advance two sweeps per pass: the workgroup
+// streams a frame of F rows over its z chunk, computing level 1 on the
+// frame's rows plus R rows on each side from a level-0 window of F + 4R rows,
+// and level 2 on the frame's rows from level 1.  Per plane: the level-0
+// window goes to LDS (one barrier), level 1 of plane m - R completes in
+// registers (ring and partial sums as in the single form) and goes to a
+// second LDS plane (one barrier), and level 2 of plane m - 2R completes and
+// is stored.  Both LDS planes are single-buffered: the barriers order each
+// write after the previous plane's reads.  Each thread owns up to 8 cells of
+// each level (flat, consecutive across threads).  A pass reads X = u1 on
+// even passes and u2 on odd ones and writes the other; with an even number
+// of fused passes per problem the state after them is in u1, as after the
+// same number of single sweeps.
+// cells per thread and level: 8 at R = 1, 4 at R = 2 (registers)
+__host__ __device__ constexpr int bp2_cells(int r) { return r == 1 ? 8 : 4; }
+constexpr int kBp2L0 = 6144;        // LDS floats of the level-0 window: <= 4096 loaded + slot padding +
+                                    // reads of cells past the frame (< 4096 + 2R*nx + R <= 5918)
+constexpr int kBp2L1 = 7168;        // level-1 plane: level-2 reads reach 3R*nx + 4095 + R <= 6827
+constexpr int kBp2Lds = kBp2L0 + kBp2L1;
+
+// Frame rows of the fused form: the level-0 window (F + 4R rows) within the
+// 8 load slots of 512 threads and the level-1 rows (F + 2R) within the
+// threads' cells; 0 when a single row does not fit.
+__host__ __device__ inline int bp2_rows(int nx, int ny, int r)
+{
+    const int fw = kBpLoads * kBpThreads / nx - 4 * r;
+    const int fc = bp2_cells(r) * kBpThreads / nx - 2 * r;
+    const int f = fw < fc ? fw : fc;
+    return f < 1 ? 0 : (f < ny ? f : ny);
+}
+
+template <int OP, int R>
+__device__ __forceinline__ void bp_tile2(const BatchProblem &q, int local, int pass, float *lds)
+{
+    constexpr int NC = bp2_cells(R);
+    const int t = threadIdx.x;
+    const int nx = q.g.nx, ny = q.g.ny, nz = q.g.nz;
+    const long long sz = q.g.sz;
+    const int ty = local % q.tiles_yf, tz = local / q.tiles_yf;
+    const int y0 = ty * q.tyf;
+    const int fr = min(q.tyf, ny - y0);
+    const int n1c = (fr + 2 * R) * nx;  // level-1 cells: rows y0-R .. y0+fr+R-1
+    const int n2c = fr * nx;            // level-2 cells: rows y0 .. y0+fr-1
+    const int z0 = tz * q.zchunkf, z1 = min(z0 + q.zchunkf, nz);
+    const float *in = (pass & 1) ? q.u2 : q.u1;
+    float *out = (pass & 1) ? q.u1 : q.u2;
+    // level-0 window rows [rlo, rhi); LDS row 0 is domain row y0 - 2R
+    const int rlo = max(y0 - 2 * R, 0), rhi = min(y0 + fr + 2 * R, ny);
+    const int nslot = ((rhi - rlo) * nx + kBpThreads - 1) / kBpThreads;
+    const int ldoff = (rlo - (y0 - 2 * R)) * nx;
+    float *L0 = lds, *L1 = lds + kBp2L0;
+
+    // x/y interior masks of the owned cells of both levels, and the stored
+    // cells of level 2 (whole interior rows: shell cells keep their value)
+    unsigned intr1 = 0, intr2 = 0, put = 0;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        const int c = t + i * kBpThreads;
+        if (c < n1c) {
+            const int r = c / nx, x = c - r * nx;
+            const int y = y0 - R + r;
+            if (x >= R && x < nx - R && y >= R && y < ny - R) intr1 |= 1u << i;
+        }
+        if (c < n2c) {
+            const int r = c / nx, x = c - r * nx;
+            const int y = y0 + r;
+            if (x >= R && x < nx - R && y >= R && y < ny - R) intr2 |= 1u << i;
+            if (y >= R && y < ny - R) put |= 1u << i;
+        }
+    }
+
+    float ring1[R][NC], part1[R][NC], ring2[R][NC], part2[R][NC], pf[kBpLoads], pf2[kBpLoads];
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+#pragma unroll
+        for (int i = 0; i < NC; ++i) ring1[j][i] = part1[j][i] = ring2[j][i] = part2[j][i] = 0.0f;
+
+    // raw buffer loads and stores, masked by offset, as in the single form
+    const __amdgpu_buffer_rsrc_t rin =
+        __builtin_amdgcn_make_buffer_rsrc((void *)in, (short)0, (int)(sz * nz * 4), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rout =
+        __builtin_amdgcn_make_buffer_rsrc((void *)out, (short)0, (int)(sz * nz * 4), 0x00020000);
+    unsigned slot_off[kBpLoads];
+#pragma unroll
+    for (int i = 0; i < kBpLoads; ++i) slot_off[i] = (unsigned)(i >= nslot) << 31;
+    auto load_plane = [&](int k, float (&d)[kBpLoads], bool valid) {
+        k = min(max(k, 0), nz - 1);
+        const unsigned base = ((unsigned)(k * sz + (long long)rlo * nx) * 4u) | ((unsigned)!valid << 31);
+#pragma unroll
+        for (int i = 0; i < kBpLoads; ++i) {
+            const unsigned off = (base | slot_off[i]) + (unsigned)(t + i * kBpThreads) * 4u;
+            d[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rin, (int)off, 0, 0));
+        }
+    };
+    const int kin_lo = z0 - 2 * R, kin_hi = z1 + 2 * R;
+    auto step = [&](int m, float (&pa)[kBpLoads]) {
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < kBpLoads; ++i)
+            L0[(i < nslot ? ldoff + i * kBpThreads : kBpLoads * kBpThreads) + t] = pa[i];
+        load_plane(m + 2, pa, m + 2 < kin_hi);
+        __syncthreads();
+        // level 1: plane m - R completes
+        const bool z1in = m - R >= R && m - R < nz - R;
+        float v1[NC];
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            const int b = R * nx + t + i * kBpThreads;
+            float s = 0.0f;
+#pragma unroll
+            for (int d = R; d >= 1; --d) s += L0[b - d];
+#pragma unroll
+            for (int d = 1; d <= R; ++d) s += L0[b + d];
+#pragma unroll
+            for (int d = R; d >= 1; --d) s += L0[b - d * nx];
+#pragma unroll
+            for (int d = 1; d <= R; ++d) s += L0[b + d * nx];
+#pragma unroll
+            for (int j = 0; j < R; ++j) s += ring1[j][i];
+            const float cen = L0[b];
+#pragma unroll
+            for (int j = 0; j < R; ++j) part1[j][i] += cen;
+            v1[i] = (z1in && ((intr1 >> i) & 1u)) ? finish<OP>(part1[0][i], ring1[0][i], q.kc) : ring1[0][i];
+#pragma unroll
+            for (int j = 0; j + 1 < R; ++j) {
+                ring1[j][i] = ring1[j + 1][i];
+                part1[j][i] = part1[j + 1][i];
+            }
+            ring1[R - 1][i] = cen;
+            part1[R - 1][i] = s;
+        }
+#pragma unroll
+        for (int i = 0; i < NC; ++i) L1[R * nx + t + i * kBpThreads] = v1[i];
+        __syncthreads();
+        // level 2: plane m - 2R completes
+        const int k = m - 2 * R;
+        const bool kst = k >= z0 && k < z1 && k >= R && k < nz - R;
+        const unsigned obase = (unsigned)(k * sz + (long long)y0 * nx) * 4u;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            const int b = 2 * R * nx + t + i * kBpThreads;
+            float s = 0.0f;
+#pragma unroll
+            for (int d = R; d >= 1; --d) s += L1[b - d];
+#pragma unroll
+            for (int d = 1; d <= R; ++d) s += L1[b + d];
+#pragma unroll
+            for (int d = R; d >= 1; --d) s += L1[b - d * nx];
+#pragma unroll
+            for (int d = 1; d <= R; ++d) s += L1[b + d * nx];
+#pragma unroll
+            for (int j = 0; j < R; ++j) s += ring2[j][i];
+            const float cen = L1[b];
+#pragma unroll
+            for (int j = 0; j < R; ++j) part2[j][i] += cen;
+            const unsigned off = (kst && ((put >> i) & 1u)) ? obase + (unsigned)(t + i * kBpThreads) * 4u
+                                                             : 0x80000000u;
+            const float res = ((intr2 >> i) & 1u) ? finish<OP>(part2[0][i], ring2[0][i], q.kc) : ring2[0][i];
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, res), rout, (int)off, 0, 0);
+#pragma unroll
+            for (int j = 0; j + 1 < R; ++j) {
+                ring2[j][i] = ring2[j + 1][i];
+                part2[j][i] = part2[j + 1][i];
+            }
+            ring2[R - 1][i] = cen;
+            part2[R - 1][i] = s;
+        }
+    };
+    auto store_none = [&]() {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) __builtin_amdgcn_raw_buffer_store_b32(0u, rout, (int)0x80000000u, 0, 0);
+    };
+    load_plane(kin_lo, pf, true);
+    store_none();
+    load_plane(kin_lo + 1, pf2, kin_lo + 1 < kin_hi);
+    store_none();
+    for (int m = kin_lo; m < kin_hi; m += 2) {
+        step(m, pf);
+        step(m + 1, pf2);
+    }
+    __syncthreads();  // the next tile rewrites the LDS planes
+}
+
 // Shell comparison of every problem: flags[p] is set to 1 by k_bp_flags,
 // then cleared by any block of k_bp_shell_eq (grid: problems x parts) that
 // finds a cell of problem p's R-thick shell whose bits differ between u1 and
@@ -315,12 +507,17 @@ __global__ __launch_bounds__(256) void k_bp_shell_eq(const BatchProblem *__restr
 
 // The plane form is instantiated per operator (a batch takes it when all its
 // problems share one), so each kernel inlines only the two radius bodies.
+// `pass` counts launches: a problem with n2 fused passes takes its single
+// sweeps from pass n2 on, sweep index pass + n2 (the state after 2*n2
+// sweeps is in u1, since n2 is even).
 template <int OP>
 __device__ __forceinline__ void bp_item(const BatchProblem *__restrict__ P, const int *__restrict__ starts,
-                                        const int *__restrict__ eqf, int np, int item, int sweep, float *lds)
+                                        const int *__restrict__ eqf, int np, int item, int pass, float *lds)
 {
     const int pi = find_problem(starts, np, item);
     const BatchProblem &q = P[pi];
+    if (pass < q.n2) return;
+    const int sweep = pass + q.n2;
     if (sweep >= q.iters) return;
     const int local = item - q.item0;
     const bool eq = eqf[pi] != 0;
@@ -337,6 +534,23 @@ __global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(OP =
     __shared__ float lds[kBpLds];
     const int tile = tile_lo + (int)xcd_remap(blockIdx.x, gridDim.x);
     if (tile < tile_hi) bp_item<OP>(P, starts, eqf, np, tile, sweep, lds);
+}
+
+// Fused items [tile_lo, tile_hi) of pass `pass`, one workgroup each: problems
+// with pass < n2 advance two sweeps.  (THESIS's division needs more
+// registers: two waves per SIMD, one workgroup per CU.)
+template <int OP>
+__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(OP == OP_THESIS ? 2 : 4))) void k_bp_fused(
+    const BatchProblem *__restrict__ P, const int *__restrict__ starts, int np, int tile_lo, int tile_hi, int pass)
+{
+    __shared__ float lds[kBp2Lds];
+    const int tile = tile_lo + (int)xcd_remap(blockIdx.x, gridDim.x);
+    if (tile >= tile_hi) return;
+    const BatchProblem &q = P[find_problem(starts, np, tile)];
+    if (pass >= q.n2) return;
+    const int local = tile - q.item0f;
+    if (q.g.r == 1) bp_tile2<OP, 1>(q, local, pass, lds);
+    else bp_tile2<OP, 2>(q, local, pass, lds);
 }
 
 template <int OP>
@@ -389,11 +603,24 @@ static int bp_rows(const st_problem &p, int want)
     return f < 1 ? 1 : f;
 }
 
+// Planes per tile of the fused form: its chunks carry 4R halo planes (2R in
+// the single form), so they are deeper than the tuned single-form chunks
+// (mostly 16 planes).  Measured on the 512-grid batch (50 sweeps, graph
+// mode): the tuned chunks 7.09 ms, whole grids in z 7.38, 64 planes 6.64,
+// 32 planes 6.49.
+static int fused_chunk(const st_problem &p)
+{
+    const int want = 32;
+    return want > p.nz ? p.nz : want;
+}
+
 st_status build_batch(const st_problem *probs, int n, const st_tile_rule *rules, int nrules,
-                      std::vector<BatchProblem> &out, int &nitems, int &maxit, bool plane_form)
+                      std::vector<BatchProblem> &out, int &nitems, int &nitemsf_out, int &maxit, bool plane_form)
 {
     out.clear();
     nitems = 0;
+    nitemsf_out = 0;
+    long long nitemsf = 0;
     maxit = 0;
     for (int i = 0; i < n; ++i) {
         const st_problem &p = probs[i];
@@ -429,20 +656,26 @@ st_status build_batch(const st_problem *probs, int n, const st_tile_rule *rules,
         b.tiles_y = (p.ny + ty - 1) / ty;
         b.zchunk = zc;
         b.item0 = nitems;
+        // fused form (chosen per problem once its shells are compared):
+        // frames as tall as the level-0 window allows
+        b.tyf = plane_form ? bp2_rows(p.nx, p.ny, p.radius) : 0;
+        b.tiles_yf = b.tyf > 0 ? (p.ny + b.tyf - 1) / b.tyf : 0;
+        b.item0f = nitemsf;
+        b.zchunkf = fused_chunk(p);
+        b.n2 = 0;
+        nitemsf += (long long)b.tiles_yf * ((p.nz + b.zchunkf - 1) / b.zchunkf);
         b.u1 = static_cast<float *>(p.u1);
         b.u2 = static_cast<float *>(p.u2);
         const long long items = (long long)b.tiles_x * b.tiles_y * ((p.nz + zc - 1) / zc);
         if (nitems + items > (1LL << 30)) return fail(ST_EINVAL, "batch too large");
         nitems += (int)items;
+        if (nitemsf > (1LL << 30)) return fail(ST_EINVAL, "batch too large");
         maxit = std::max(maxit, p.iterations);
         out.push_back(b);
     }
+    nitemsf_out = (int)nitemsf;
     return ST_OK;
 }
-
-// Bytes (both buffers) of one run of problems in graph mode: under the
-// 256 MiB last-level cache with room to spare.
-constexpr long long kBatchRunBytes = 160LL << 20;
 
 // The instantiated graph of the last batch dispatched in graph mode on each
 // (device, stream): a dispatch whose table, workspace and launch shape are
@@ -488,19 +721,24 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
     for (int i = 0; i < n; ++i) plane_form = plane_form && bp_fits(probs[i]) && probs[i].op == probs[0].op;
     const bool thesis = probs[0].op == ST_OP_THESIS;
     std::vector<BatchProblem> table;
-    int nitems = 0, maxit = 0;
-    st_status st = build_batch(probs, n, rules, nrules, table, nitems, maxit, plane_form);
+    int nitems = 0, nitemsf = 0, maxit = 0;
+    st_status st = build_batch(probs, n, rules, nrules, table, nitems, nitemsf, maxit, plane_form);
     if (st != ST_OK) return st;
-    // device copy: the problem table, then each problem's first item, then
-    // room for the shell-comparison flags
+    // device copy: the problem table, each problem's first item in the
+    // single and in the fused list, then the shell-comparison flags
     void *ws = nullptr;
     const size_t tbytes = (table.size() * sizeof(BatchProblem) + 255) & ~(size_t)255;
     const size_t sbytes = (table.size() * sizeof(int) + 255) & ~(size_t)255;
     const size_t fbytes = table.size() * sizeof(int);
-    std::vector<unsigned char> host(tbytes + sbytes, 0);
-    std::memcpy(host.data(), table.data(), table.size() * sizeof(BatchProblem));
-    for (size_t i = 0; i < table.size(); ++i)
-        std::memcpy(host.data() + tbytes + i * sizeof(int), &table[i].item0, sizeof(int));
+    std::vector<unsigned char> host(tbytes + 2 * sbytes, 0);
+    auto pack = [&]() {
+        std::memcpy(host.data(), table.data(), table.size() * sizeof(BatchProblem));
+        for (size_t i = 0; i < table.size(); ++i) {
+            std::memcpy(host.data() + tbytes + i * sizeof(int), &table[i].item0, sizeof(int));
+            std::memcpy(host.data() + tbytes + sbytes + i * sizeof(int), &table[i].item0f, sizeof(int));
+        }
+    };
+    pack();
     if ((st = workspace_slot(1, host.size() + fbytes, s, &ws)) != ST_OK) return st;
     if ((st = hip_check(hipMemcpyAsync(ws, host.data(), host.size(), hipMemcpyHostToDevice, s),
                         "batch table copy")) != ST_OK)
@@ -508,10 +746,11 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
     if ((st = hip_check(hipStreamSynchronize(s), "batch table sync")) != ST_OK) return st;
     const BatchProblem *P = static_cast<const BatchProblem *>(ws);
     const int *starts = reinterpret_cast<const int *>(static_cast<unsigned char *>(ws) + tbytes);
-    int *eqf = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + tbytes + sbytes);
+    const int *startsf = reinterpret_cast<const int *>(static_cast<unsigned char *>(ws) + tbytes + sbytes);
+    int *eqf = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + tbytes + 2 * sbytes);
     const int *ceqf = eqf;
     // plane form: compare the problems' shells first (on the stream, inside
-    // the timed region and the graph)
+    // the timed region)
     // parts per problem: about 8192 shell cells per block of the largest shell
     long long max_shell = 1;
     for (int i = 0; i < n; ++i)
@@ -551,28 +790,48 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
         e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), plane_form ? args_bp : args_march, 0, s);
         if (e != hipSuccess) return hip_check(e, "batched cooperative launch");
     } else {
-        // capture one launch per sweep, replay as one graph; the
-        // instantiated graph is kept for the next dispatch of the same batch
+        // Graph mode.  The shells are compared first and the flags read back,
+        // so that every problem with equal shells takes fused passes of two
+        // sweeps (n2 of them, an even number, then iters mod 4 single
+        // sweeps) and the rest single sweeps; then one launch per pass (a
+        // fused and/or a single-sweep launch) is captured and replayed as one
+        // graph.  The instantiated graph is kept for the next dispatch with
+        // the same table (n2 included), workspace and launch shape.
+        if (elapsed_ms) (void)hipEventRecord(e0, s);
+        bool any_fused = false;
+        if (plane_form) {
+            shell_check();
+            std::vector<int> flags(n, 0);
+            hipError_t e = hipMemcpyAsync(flags.data(), eqf, fbytes, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return hip_check(e, "batch shell flags");
+            for (int i = 0; i < n; ++i) {
+                BatchProblem &b = table[i];
+                b.n2 = (flags[i] && b.tyf > 0) ? (b.iters - b.iters % 4) / 2 : 0;
+                any_fused = any_fused || b.n2 > 0;
+            }
+            if (any_fused) {
+                pack();
+                e = hipMemcpyAsync(ws, host.data(), tbytes, hipMemcpyHostToDevice, s);
+                if (e != hipSuccess) return hip_check(e, "batch table update");
+            }
+        }
         const dim3 block(plane_form ? kBpThreads : 256);
         int dev = 0;
         (void)hipGetDevice(&dev);
-        std::vector<unsigned char> key(host.size() + sizeof(void *) + 6 * sizeof(int));
+        std::vector<unsigned char> key(host.size() + sizeof(void *) + 7 * sizeof(int));
         {
             unsigned char *k = key.data();
             std::memcpy(k, host.data(), host.size());
             k += host.size();
             std::memcpy(k, &ws, sizeof(void *));
             k += sizeof(void *);
-            const int meta[6] = {n, nitems, maxit, (int)plane_form, (int)thesis, groups};
+            const int meta[7] = {n, nitems, nitemsf, maxit, (int)plane_form, (int)thesis, groups};
             std::memcpy(k, meta, sizeof meta);
         }
         std::lock_guard<std::mutex> lock(g_graph_mutex);
         BatchGraph &cached = g_graphs[std::make_pair(dev, s)];
         hipError_t e = hipSuccess;
-        // elapsed time starts before a capture, so that a first dispatch
-        // counts its capture and instantiation (host work while the stream
-        // waits) as well as the replay
-        if (elapsed_ms) (void)hipEventRecord(e0, s);
         if (!cached.exec || cached.key != key) {
             if (cached.exec) {
                 (void)hipGraphExecDestroy(cached.exec);
@@ -582,37 +841,30 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
             hipGraph_t graph = nullptr;
             e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
             if (e != hipSuccess) return hip_check(e, "begin capture");
-            shell_check();
-            // Problems in runs whose two buffers together fit in the
-            // last-level cache: a run takes all its sweeps before the next
-            // one starts, so after its first sweep its planes come from the
-            // 256 MiB cache rather than from HBM.
-            std::vector<int> cut(1, 0);  // first problem of each run
-            {
-                long long acc = 0;
-                for (int i = 0; i < n; ++i) {
-                    const long long b = 8LL * probs[i].nx * probs[i].ny * probs[i].nz;
-                    if (acc > 0 && acc + b > kBatchRunBytes) {
-                        cut.push_back(i);
-                        acc = 0;
-                    }
-                    acc += b;
+            // passes: problem i runs n2 fused passes, then iters - 2*n2
+            // single sweeps (pass index p >= n2 is its sweep p + n2)
+            int passes = 0;
+            for (const BatchProblem &b : table) passes = std::max(passes, b.iters - b.n2);
+            for (int p = 0; p < passes; ++p) {
+                bool fused = false, single = false;
+                for (const BatchProblem &b : table) {
+                    fused = fused || p < b.n2;
+                    single = single || (p >= b.n2 && p < b.iters - b.n2);
                 }
-                cut.push_back(n);
-            }
-            for (size_t g = 0; g + 1 < cut.size(); ++g) {
-                const int lo = table[cut[g]].item0;
-                const int hi = cut[g + 1] < n ? table[cut[g + 1]].item0 : nitems;
-                int its = 0;
-                for (int i = cut[g]; i < cut[g + 1]; ++i) its = std::max(its, probs[i].iterations);
-                const int span = hi - lo;
-                const dim3 gg(plane_form ? span : (span + 3) / 4);
-                for (int sw = 0; sw < its && span > 0; ++sw) {
+                if (fused && nitemsf > 0) {
+                    const dim3 gf(nitemsf);
+                    if (thesis)
+                        hipLaunchKernelGGL(k_bp_fused<OP_THESIS>, gf, block, 0, s, P, startsf, n, 0, nitemsf, p);
+                    else
+                        hipLaunchKernelGGL(k_bp_fused<OP_JACOBI>, gf, block, 0, s, P, startsf, n, 0, nitemsf, p);
+                }
+                if (single && nitems > 0) {
+                    const dim3 gg(plane_form ? nitems : (nitems + 3) / 4);
                     if (plane_form && thesis)
-                        hipLaunchKernelGGL(k_bp_sweep<OP_THESIS>, gg, block, 0, s, P, starts, ceqf, n, lo, hi, sw);
+                        hipLaunchKernelGGL(k_bp_sweep<OP_THESIS>, gg, block, 0, s, P, starts, ceqf, n, 0, nitems, p);
                     else if (plane_form)
-                        hipLaunchKernelGGL(k_bp_sweep<OP_JACOBI>, gg, block, 0, s, P, starts, ceqf, n, lo, hi, sw);
-                    else hipLaunchKernelGGL(k_batch_sweep, gg, block, 0, s, P, starts, n, lo, hi, sw);
+                        hipLaunchKernelGGL(k_bp_sweep<OP_JACOBI>, gg, block, 0, s, P, starts, ceqf, n, 0, nitems, p);
+                    else hipLaunchKernelGGL(k_batch_sweep, gg, block, 0, s, P, starts, n, 0, nitems, p);
                 }
             }
             e = hipStreamEndCapture(s, &graph);
