@@ -133,7 +133,12 @@ typedef struct st_params {
                              radius 1, nx a multiple of 4 up to 1024; two
                              fused sweeps per pass whether or not the two
                              buffers hold the same shell; block_y = waves
-                             per frame, coarsen_y = rows per wave),
+                             per frame, coarsen_y = rows per wave); in 2D
+                             (the only COOP form there) 512-column segments
+                             per wave, four fused sweeps per pass when both
+                             buffers hold the same shell (else single
+                             sweeps), nx a multiple of 8, block_y = rows per
+                             chunk,
                              4 = row frames of the SMEM_REGISTER row form, one
                              sweep per grid barrier (3D F32/F64, radius 1..4,
                              nx a multiple of 4).

@@ -86,7 +86,8 @@ def test_invalid_knobs(st):
             coarsen_x=4)                         # LDS over 160 KiB
     _raises(st, 5, variant="smem_register", radius=17, dim=64, heuristic=1)
     _raises(st, 5, variant="coop", radius=5, dim=64, heuristic=1)
-    _raises(st, 5, variant="coop", dimensions=2, dim=64, heuristic=1)
+    _raises(st, 5, variant="coop", dimensions=2, dim=64, radius=2, heuristic=1)  # 2D streaming: radius 1
+    _raises(st, 5, variant="coop", dimensions=2, dim=60, heuristic=1)            # nx not a multiple of 8
 
 
 def test_launch_config_shapes(st):

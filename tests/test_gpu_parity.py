@@ -530,6 +530,51 @@ def test_coop_stream_form_rejects_unsupported(gpu):
             st.check_early_exit(p)
 
 
+# 2D streaming form: four fused sweeps per pass, 512-column segments (496
+# stored), chunks of rows; equal shells fuse, different ones run single sweeps
+STREAM2D_SHAPES = [
+    ((512, 300), {}),
+    ((1000, 37), {}),                                   # two segments, the last one short
+    ((16, 20), {}),                                     # one segment, mostly outside the grid
+    ((1504, 260), {"block_y": 24, "heuristic": 0}),     # four segments, 11 chunks of 24 rows
+    ((2000, 9), {"block_y": 5, "heuristic": 0}),        # chunks shorter than the 4 halo rows
+]
+
+
+@pytest.mark.parametrize("shape,knobs", STREAM2D_SHAPES, ids=str)
+@pytest.mark.parametrize("op", [0, 1])
+@pytest.mark.parametrize("init", ["seeded", "reference"])
+def test_coop_stream2d_form(gpu, oracle, shape, knobs, op, init):
+    st = gpu
+    nx, ny = shape
+    if init == "reference":
+        u1 = oracle.init_reference(nx, ny, 1, 2)
+        u2 = np.zeros_like(u1)
+    else:
+        u1 = oracle.init_seeded(nx, ny, 1, 2, 1, 0x5EED0070)
+        rng = np.random.default_rng(nx + ny)
+        u1 = np.where(u1 == 0, rng.random(u1.shape), u1).astype(np.float32)
+        u2 = u1.copy()
+    kw = dict(heuristic=1)
+    kw.update(knobs)
+    if kw["heuristic"] == 0:
+        kw["block_x"] = 512
+    for its in (1, 4, 7, 8, 9, 16, 17, 23):
+        _announce("coop stream2d form", shape, knobs, op, init, its)
+        want = oracle.run(u1.copy(), u2.copy(), 2, 1, op, its).copy()
+        p = st.StencilParams.make(nx=nx, ny=ny, nz=1, dimensions=2, radius=1, iterations=its, op=op,
+                                  variant="coop", **kw)
+        cfg = st.launch_config(p)
+        assert cfg["launches"] == 2 * (its // 8) + its % 8
+        (got, other), in_u2 = run_gpu_both(st, p, u1, u2)
+        assert in_u2 == bool(its % 2)
+        assert_bitwise(got, want, f"stream2d {shape} {knobs} op={op} {init} its={its}")
+        shell = np.ones(u1.shape, bool)
+        shell[..., 1:-1, 1:-1] = False
+        orig = u1 if in_u2 else u2
+        assert_bitwise(other[shell], orig[shell], f"stream2d shell {shape} op={op} {init} its={its}")
+
+
 # COOP row form: the row kernel, every sweep in one cooperative launch with a
 # grid barrier after each (or one launch per sweep)
 COOP_ROWS_CASES = [
@@ -774,7 +819,8 @@ def test_coop_cooperative_launch_timed_on_null_stream(gpu, oracle, form):
     (3, 1024, 100, "coop"),          # one row of 16 cells per lane
     (3, 768, 100, "smem_register"),  # a quarter of those lanes idle: the row kernel
     (2, 256, 64, "base"),
-    (2, 4096, 64, "smem_register"),
+    (2, 4096, 64, "smem_register"),  # both fields under 192 MiB
+    (2, 8192, 64, "coop"),           # above: the 2D streaming form
 ])
 def test_auto_choice_by_size(gpu, dims, n, its, want):
     st = gpu

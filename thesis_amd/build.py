@@ -35,7 +35,7 @@ SOURCES = [
     "sweep_base.hip",
     "sweep_march.hip",
     "sweep_coop.hip",
-    "sweep_tblock.hip", "sweep_stream.hip", "sweep_rows.hip",
+    "sweep_tblock.hip", "sweep_stream.hip", "sweep_stream2d.hip", "sweep_rows.hip",
     "batched.hip",
     "unet.hip",
     "init_reduce.hip",

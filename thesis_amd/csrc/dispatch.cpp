@@ -219,6 +219,13 @@ static st_status resolve(const st_params *p, Cfg &c, bool mixed)
             c.cx = 1;
             c.cy = 0;
             c.per_pass = 1;
+        } else if (c.dims == 2 && c.iters >= 8 && c.form == 0 && (c.tsteps == 0 || c.tsteps == 4) &&
+                   stream2d_eligible(c) && 2 * cells * es > stream_bytes) {
+            // 2D fields above the last-level cache: four fused sweeps per pass
+            c.variant = ST_V_COOP;
+            c.cx = 1;
+            c.cy = 0;
+            c.per_pass = 1;
         } else if (cells <= small) {
             c.variant = ST_V_BASE;
             c.cx = 1;
@@ -299,10 +306,14 @@ st_status stencil_launch_config(const st_params *p, st_launch_info *info)
             if (n && (n & 1) != (c.iters & 1)) ++n;
             if (c.kind == 2) n = c.iters / c.tsteps;
             // streaming form per pass: n2 fused passes, then iters mod 4
-            // single ones (the band fix-up kernels are not counted)
+            // single ones
             if (c.kind == 3) n = c.iters > 0 ? (c.per_pass ? (c.iters - c.iters % 4) / 2 + c.iters % 4 : 1) : 0;
             if (c.kind == 5) n = c.iters;
             info->launches = n ? (c.per_pass ? n : 1) : 0;
+            // 2D streaming form, always one launch per pass: 2*(iters/8)
+            // passes of four sweeps, then iters mod 8 single sweeps (with
+            // equal shells)
+            if (c.kind == 6) info->launches = 2 * (c.iters / 8) + c.iters % 8;
         } else {
             info->launches = c.iters;
         }

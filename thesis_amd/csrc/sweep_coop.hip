@@ -212,7 +212,13 @@ static long long coop_lds(const Cfg &c, int t)
 // chunk, coarsen_y is unused and the workgroup is 256 threads.
 st_status resolve_coop(Cfg &c)
 {
-    if (c.dims != 3) return fail(ST_EUNSUPPORTED, "COOP is implemented for 3D grids");
+    if (c.dims == 2) {
+        // 2D: the streaming form (four fused sweeps per pass)
+        if (c.form != 0 && c.form != 3)
+            return fail(ST_EUNSUPPORTED, "COOP in 2D has the streaming form (temporal_form 0 or 3) only");
+        return resolve_stream2d(c);
+    }
+    if (c.dims != 3) return fail(ST_EUNSUPPORTED, "COOP is implemented for 2D and 3D grids");
     if (c.r > 4) return fail(ST_EUNSUPPORTED, "COOP is built for radius 1..4");
     const bool heur = c.heuristic || c.bx <= 0;
     // streaming form: requested, or the library's choice for heuristic
@@ -298,6 +304,21 @@ st_status run_coop(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2)
     if (c.kind == 2) return run_tblock(c, u1, u2, s, in_u2);
     if (c.kind == 3) return run_stream(c, u1, u2, s, in_u2);
     if (c.kind == 5) return run_rows_coop(c, u1, u2, s, in_u2);
+    if (c.kind == 6) {
+        // sweeps outside fused passes: the library's per-sweep 2D kernel
+        Cfg sc = c;
+        sc.variant = ST_V_SMEM_REGISTER;
+        sc.heuristic = 1;
+        sc.bx = sc.by = sc.bz = 0;
+        sc.cx = 1;
+        sc.cy = 0;
+        sc.form = 0;
+        sc.tsteps = 0;
+        sc.zchunk = 0;
+        st_status st = resolve_march(sc);
+        if (st != ST_OK) return st;
+        return run_stream2d(c, sc, u1, u2, s, in_u2);
+    }
     const void *fn = pick_coop(c);
     if (!fn) return fail(ST_EUNSUPPORTED, "no COOP kernel for radius %d", c.r);
     CoopArgs a;

@@ -1,0 +1,280 @@
+// COOP family, 2D streaming form: T fused sweeps per HBM pass of a 2D grid
+// (radius 1, FP32), for grids far larger than the last-level cache (the
+// reference evaluates 2D up to 32768^2, scripts/evaluate_stencils.py:71-76).
+//
+// Each wave owns a segment of 512 consecutive columns (8 per lane) and
+// marches it along y over a chunk of rows, carrying T time levels at once:
+// when row m of level 0 arrives, level l completes row m - l from the three
+// rows m-l-1 .. m-l+1 of level l-1 and passes it on, so one HBM pass reads
+// the field once and writes it once for T sweeps.  Per level and row a lane
+// holds the previous row (the j-1 term and the centre of the row it
+// completes) and the row's running sum (i-prev, i-next, j-prev terms in the
+// reference's order, src/stencil_accumulators.cu:46-74; the j-next term
+// completes it one row later).  x neighbours inside a lane are its own
+// values, across lanes one DPP wave rotate.  No LDS, no barriers: waves are
+// independent.
+//
+// A level loses one valid column at each segment end (the rotate wraps
+// there), so segments overlap: a segment starts 8 columns left of the 496
+// columns it stores (T <= 8), and chunks overlap by T rows on each side.
+// Shell cells keep their value at every level, which is exact when both
+// buffers hold the same shell; the host runs single sweeps otherwise.
+#include <algorithm>
+
+#include "st_device.hpp"
+#include "st_internal.hpp"
+
+namespace st {
+
+constexpr int kS2Cols = 512;   // columns per segment (64 lanes x 8)
+constexpr int kS2Core = 496;   // stored columns per segment
+constexpr int kS2Lead = 8;     // columns left of the stored ones
+
+struct S2Args {
+    int nx, ny;
+    int nseg;        // segments per row band
+    int chunk;       // rows per chunk
+    int nchunks;
+    int nwaves;      // nseg * nchunks
+};
+
+typedef float V4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float s2_prev_lane(float v)  // lane i <- lane i-1 (rotating)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x13C, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float s2_next_lane(float v)  // lane i <- lane i+1 (rotating)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134, 0xF, 0xF, true));
+}
+
+template <int OP, int T>
+__global__ __launch_bounds__(256) void k_stream2d(const float *__restrict__ X, float *__restrict__ Y, S2Args a,
+                                                  float kc)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)xcd_remap(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6);
+    if (wave >= a.nwaves) return;
+    const int seg = wave % a.nseg, ch = wave / a.nseg;
+    const int x0 = seg * kS2Core;                     // first stored column
+    const int c0 = x0 - kS2Lead + 8 * lane;            // this lane's first column
+    const bool inx = c0 >= 0 && c0 + 8 <= a.nx;        // lane inside the row
+    const int cl = inx ? c0 : 0;                       // load column
+    const bool store_lane = lane >= 1 && lane <= 62 && inx;
+    const int y0 = ch * a.chunk, y1 = min(y0 + a.chunk, a.ny);
+    // column interior flags of the lane's 8 cells
+    unsigned xin = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        if (c0 + c >= 1 && c0 + c < a.nx - 1) xin |= 1u << c;
+
+    float prev[T][8], part[T][8];
+#pragma unroll
+    for (int l = 0; l < T; ++l)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) prev[l][c] = part[l][c] = 0.0f;
+
+    auto load_row = [&](int m, float (&d)[8]) {
+        m = min(max(m, 0), a.ny - 1);
+        const float *p = X + (long long)m * a.nx + cl;
+        const V4 q0 = *reinterpret_cast<const V4 *>(p);
+        const V4 q1 = *reinterpret_cast<const V4 *>(p + 4);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            d[j] = q0[j];
+            d[4 + j] = q1[j];
+        }
+    };
+
+    const int m_lo = y0 - T, m_hi = y1 + T;  // level-0 rows marched
+    // one step: level-0 row m arrives in `in`; level l completes row m - l
+    auto step = [&](int m, float (&in)[8]) {
+        __builtin_amdgcn_sched_barrier(0);
+        float v[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) v[c] = in[c];
+#pragma unroll
+        for (int l = 0; l < T; ++l) {
+            const int q = m - l - 1;  // row level l+1 completes
+            const bool yin = q >= 1 && q < a.ny - 1;
+            const float xp = s2_prev_lane(v[7]);
+            const float xn = s2_next_lane(v[0]);
+            float out[8], np[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                // row q: its sum plus the j-next term (row q+1 = v)
+                const float fin = finish<OP>(part[l][c] + v[c], prev[l][c], kc);
+                out[c] = (yin && ((xin >> c) & 1u)) ? fin : prev[l][c];
+                // row q+1: i-prev, i-next, j-prev
+                float s = 0.0f;
+                s += c > 0 ? v[c - 1] : xp;
+                s += c < 7 ? v[c + 1] : xn;
+                s += prev[l][c];
+                np[c] = s;
+            }
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                prev[l][c] = v[c];
+                part[l][c] = np[c];
+                v[c] = out[c];
+            }
+        }
+        // v: level T, row m - T
+        const int q = m - T;
+        if (store_lane && q >= y0 && q < y1 && q >= 1 && q < a.ny - 1) {
+            // whole rows: the x-shell cells hold their unchanged value
+            float *p = Y + (long long)q * a.nx + c0;
+            V4 s0, s1;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                s0[j] = v[j];
+                s1[j] = v[4 + j];
+            }
+            __builtin_nontemporal_store(s0, reinterpret_cast<V4 *>(p));
+            __builtin_nontemporal_store(s1, reinterpret_cast<V4 *>(p + 4));
+        }
+    };
+
+    // two rows in flight: the loop alternates the buffers, so no register
+    // copy forces an early wait
+    float pa[8], pb[8];
+    load_row(m_lo, pa);
+    load_row(m_lo + 1, pb);
+    for (int m = m_lo; m < m_hi; m += 2) {
+        float ca[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) ca[c] = pa[c];
+        if (m + 2 < m_hi) load_row(m + 2, pa);
+        step(m, ca);
+        if (m + 1 >= m_hi) break;
+        float cb[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) cb[c] = pb[c];
+        if (m + 3 < m_hi) load_row(m + 3, pb);
+        step(m + 1, cb);
+    }
+}
+
+// *flag = 0 when a cell of the 1-thick shell of the 2D grid (rows 0 and
+// ny-1, columns 0 and nx-1) differs between u1 and u2 (bitwise).
+__global__ __launch_bounds__(256) void k_shell_eq2d(const unsigned *__restrict__ u1, const unsigned *__restrict__ u2,
+                                                    int nx, int ny, int *__restrict__ flag)
+{
+    const long long nrow = 2LL * nx, n = nrow + 2LL * (ny - 2);
+    int ok = 1;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) {
+        long long idx;
+        if (i < nrow) {
+            idx = i < nx ? i : (long long)(ny - 1) * nx + (i - nx);
+        } else {
+            const long long j = i - nrow;
+            idx = (1 + j / 2) * (long long)nx + ((j & 1) ? nx - 1 : 0);
+        }
+        ok &= u1[idx] == u2[idx];
+    }
+    ok = __syncthreads_and(ok);
+    if (threadIdx.x == 0 && !ok) *flag = 0;
+}
+
+__global__ void k_flag_set(int *f) { if (threadIdx.x == 0) *f = 1; }
+
+// ---------------------------------------------------------------- host side
+
+constexpr int kS2Steps = 4;  // sweeps fused per pass
+
+bool stream2d_eligible(const Cfg &c)
+{
+    return c.dims == 2 && c.dtype == ST_F32 && c.r == 1 && c.nx % 8 == 0 && c.nx >= 16 && c.ny >= 3;
+}
+
+st_status resolve_stream2d(Cfg &c)
+{
+    if (!stream2d_eligible(c))
+        return fail(ST_EUNSUPPORTED, "2D streaming form needs FP32, radius 1, nx a multiple of 8 (>= 16) and "
+                                     "ny >= 3");
+    c.kind = 6;
+    c.tsteps = kS2Steps;
+    c.cx = 8;
+    c.cy = 1;
+    c.bx = kS2Cols;
+    c.by = 1;
+    c.bz = 1;
+    // rows per chunk: block_y with HEURISTIC=0, else 256 (chunks overlap by
+    // 2T rows; 256 keeps that at 3% while giving enough waves)
+    int chunk = (!c.heuristic && c.by > 0) ? c.by : 256;
+    if (chunk > c.ny) chunk = c.ny;
+    c.zchunk = chunk;
+    const int nseg = (c.nx + kS2Core - 1) / kS2Core;
+    const int nch = (c.ny + chunk - 1) / chunk;
+    c.gx = (nseg * nch + 3) / 4;
+    c.gy = c.gz = 1;
+    c.lds = 0;
+    return ST_OK;
+}
+
+static const void *stream2d_fn(int op)
+{
+    return op == OP_THESIS ? reinterpret_cast<const void *>(&k_stream2d<OP_THESIS, kS2Steps>)
+                           : reinterpret_cast<const void *>(&k_stream2d<OP_JACOBI, kS2Steps>);
+}
+
+// Passes: n4 of kS2Steps fused sweeps (an even number of passes, so the
+// state after them is in u1 as after the same number of single sweeps),
+// then the remaining iters - 4*n4 sweeps one at a time with the per-sweep
+// kernel of `single`; with different shells, or when the shells cannot be
+// compared on the host (graph capture, the no-alloc entry point), every
+// sweep runs singly.
+st_status run_stream2d(const Cfg &c, const Cfg &single, void *u1, void *u2, hipStream_t s, int *in_u2)
+{
+    *in_u2 = c.iters & 1;
+    if (c.iters <= 0) return ST_OK;
+    int n4 = 2 * (c.iters / (2 * kS2Steps));
+    if (n4 > 0) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        bool same = false;
+        if (!workspace_is_fixed() && hipStreamIsCapturing(s, &cs) == hipSuccess &&
+            cs == hipStreamCaptureStatusNone) {
+            void *ws = nullptr;
+            st_status st = workspace_slot(2, sizeof(int), s, &ws);
+            if (st != ST_OK) return st;
+            int *f = static_cast<int *>(ws);
+            const long long n = 2LL * c.nx + 2LL * (c.ny - 2);
+            const int blocks = (int)std::max<long long>(1, std::min<long long>(1024, (n + 2047) / 2048));
+            hipLaunchKernelGGL(k_flag_set, dim3(1), dim3(64), 0, s, f);
+            hipLaunchKernelGGL(k_shell_eq2d, dim3(blocks), dim3(256), 0, s, static_cast<const unsigned *>(u1),
+                               static_cast<const unsigned *>(u2), c.nx, c.ny, f);
+            int h = 0;
+            hipError_t e = hipMemcpyAsync(&h, f, sizeof h, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return hip_check(e, "2D shell comparison read-back");
+            same = h != 0;
+        }
+        if (!same) n4 = 0;
+    }
+    S2Args a;
+    a.nx = c.nx;
+    a.ny = c.ny;
+    a.nseg = (c.nx + kS2Core - 1) / kS2Core;
+    a.chunk = c.zchunk;
+    a.nchunks = (c.ny + c.zchunk - 1) / c.zchunk;
+    a.nwaves = a.nseg * a.nchunks;
+    const float kf = c.op == OP_THESIS ? 4.0f : (float)c.weight;
+    const void *fn = stream2d_fn(c.op);
+    for (int p = 0; p < n4; ++p) {
+        const float *x = static_cast<const float *>((p & 1) ? u2 : u1);
+        float *y = static_cast<float *>((p & 1) ? u1 : u2);
+        void *args[] = {&x, &y, &a, const_cast<float *>(&kf)};
+        hipError_t e = hipLaunchKernel(fn, dim3((a.nwaves + 3) / 4), dim3(256), args, 0, s);
+        if (e != hipSuccess) return hip_check(e, "2D stream pass launch");
+    }
+    for (int i = kS2Steps * n4; i < c.iters; ++i) {
+        const void *in = (i & 1) ? u2 : u1;
+        void *out = (i & 1) ? u1 : u2;
+        st_status st = launch_march(single, in, out, s);
+        if (st != ST_OK) return st;
+    }
+    return ST_OK;
+}
+
+}  // namespace st
