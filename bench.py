@@ -316,13 +316,14 @@ def mixed_leg(st, np, repeats: int = 3, with_cpu: bool = True):
 
 
 def size_legs(st, np, repeats: int = 3):
-    """The reference's own evaluation sizes (scripts/evaluate_stencils.py:71-76):
-    3D 1024^3 and 2D 32768^2, R=1 FP32 JACOBI, seeded boundary, 20 sweeps
-    each, AUTO's choice of kernel; GCells/s and the per-sweep algorithmic
-    HBM fraction (4 B x (cells + interior cells) per sweep)."""
+    """The reference's own evaluation sizes and sweep count
+    (scripts/evaluate_stencils.py:9,71-76): 3D 1024^3 and 2D 32768^2, 8
+    sweeps, R=1 FP32 JACOBI, seeded boundary, AUTO's choice of kernel;
+    GCells/s and the per-sweep algorithmic HBM fraction (4 B x (cells +
+    interior cells) per sweep)."""
     out = {}
     for name, dims, n in (("sweep1024", 3, 1024), ("sweep2d_32768", 2, 32768)):
-        its = 20
+        its = 8
         p = st.StencilParams.make(nx=n, ny=n, nz=n if dims == 3 else 1, dimensions=dims, radius=1,
                                   iterations=its, op=st.OP_JACOBI, variant="auto", heuristic=1)
         cfg = st.launch_config(p)
@@ -484,9 +485,10 @@ def main():
                     help="skip the mixed384 and batched legs of the headline line")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--workload", default="sweep512",
-                    choices=["sweep512", "batched", "mixed384"],
+                    choices=["sweep512", "batched", "mixed384", "sizes"],
                     help="sweep512: the headline 512^3 sweep with its legs (default); batched / "
-                         "mixed384: that leg alone, as its own line")
+                         "mixed384 / sizes (1024^3 and 32768^2, 8 sweeps): that leg alone, as its "
+                         "own line")
     ap.add_argument("--probe", action="store_true",
                     help="short run for counter collection: one streaming copy of a 512^3 "
                          "field, 2 sweeps of each single-sweep family and of the LDS-ring "
@@ -525,6 +527,13 @@ def main():
     if args.workload == "mixed384":
         out = mixed_leg(st, np, max(1, min(args.steps, 5)), with_cpu=not args.no_cpu)
         out["n_gpus"] = 1
+        print(json.dumps(out), flush=True)
+        return
+
+    if args.workload == "sizes":
+        out = {"metric": "GCells/s at the reference's evaluation sizes", "unit": "GCells/s",
+               "higher_is_better": True, "dtype": "f32", "n_gpus": 1,
+               "legs": size_legs(st, np, max(1, min(args.steps, 5)))}
         print(json.dumps(out), flush=True)
         return
 

@@ -135,9 +135,10 @@ typedef struct st_params {
                              buffers hold the same shell; block_y = waves
                              per frame, coarsen_y = rows per wave); in 2D
                              (the only COOP form there) 512-column segments
-                             per wave, four fused sweeps per pass when both
-                             buffers hold the same shell (else single
-                             sweeps), nx a multiple of 8, block_y = rows per
+                             per wave, six and four fused sweeps per pass
+                             when both buffers hold the same shell (else
+                             single sweeps; temporal_steps 4 keeps passes of
+                             four), nx a multiple of 8, block_y = rows per
                              chunk,
                              4 = row frames of the SMEM_REGISTER row form, one
                              sweep per grid barrier (3D F32/F64, radius 1..4,

@@ -533,12 +533,32 @@ def test_coop_stream_form_rejects_unsupported(gpu):
 # 2D streaming form: four fused sweeps per pass, 512-column segments (496
 # stored), chunks of rows; equal shells fuse, different ones run single sweeps
 STREAM2D_SHAPES = [
-    ((512, 300), {}),
+    ((512, 300), {"temporal_steps": 4}),                # passes of four only
     ((1000, 37), {}),                                   # two segments, the last one short
     ((16, 20), {}),                                     # one segment, mostly outside the grid
     ((1504, 260), {"block_y": 24, "heuristic": 0}),     # four segments, 11 chunks of 24 rows
     ((2000, 9), {"block_y": 5, "heuristic": 0}),        # chunks shorter than the 4 halo rows
+    ((512, 300), {}),                                   # passes of six, then of four
+    ((1504, 260), {"temporal_steps": 6, "block_y": 24, "heuristic": 0}),
 ]
+
+
+def _stream2d_launches(its, tmax):
+    """Passes of tmax and 4 sweeps (an even count, the most sweeps fused,
+    then the fewest passes), then single sweeps."""
+    best = (4 * 2 * (its // 8), 0, 2 * (its // 8))
+    if tmax != 4:
+        for a in range(1, its // tmax + 1):
+            b = (its - tmax * a) // 4
+            if (a + b) % 2:
+                b -= 1
+            if b < 0:
+                continue
+            fused = tmax * a + 4 * b
+            if (fused, a) > (best[0], best[1]):
+                best = (fused, a, b)
+    fused, a, b = best
+    return a + b + its - fused
 
 
 @pytest.mark.parametrize("shape,knobs", STREAM2D_SHAPES, ids=str)
@@ -559,13 +579,13 @@ def test_coop_stream2d_form(gpu, oracle, shape, knobs, op, init):
     kw.update(knobs)
     if kw["heuristic"] == 0:
         kw["block_x"] = 512
-    for its in (1, 4, 7, 8, 9, 16, 17, 23):
+    for its in (1, 4, 7, 8, 9, 12, 16, 17, 23, 28):
         _announce("coop stream2d form", shape, knobs, op, init, its)
         want = oracle.run(u1.copy(), u2.copy(), 2, 1, op, its).copy()
         p = st.StencilParams.make(nx=nx, ny=ny, nz=1, dimensions=2, radius=1, iterations=its, op=op,
                                   variant="coop", **kw)
         cfg = st.launch_config(p)
-        assert cfg["launches"] == 2 * (its // 8) + its % 8
+        assert cfg["launches"] == _stream2d_launches(its, knobs.get("temporal_steps", 6))
         (got, other), in_u2 = run_gpu_both(st, p, u1, u2)
         assert in_u2 == bool(its % 2)
         assert_bitwise(got, want, f"stream2d {shape} {knobs} op={op} {init} its={its}")
@@ -819,8 +839,9 @@ def test_coop_cooperative_launch_timed_on_null_stream(gpu, oracle, form):
     (3, 1024, 100, "coop"),          # one row of 16 cells per lane
     (3, 768, 100, "smem_register"),  # a quarter of those lanes idle: the row kernel
     (2, 256, 64, "base"),
-    (2, 4096, 64, "smem_register"),  # both fields under 192 MiB
-    (2, 8192, 64, "coop"),           # above: the 2D streaming form
+    (2, 1024, 64, "base"),           # launch-bound
+    (2, 2048, 64, "coop"),           # above: the 2D streaming form
+    (2, 4096, 7, "smem_register"),   # too few sweeps for two fused passes
 ])
 def test_auto_choice_by_size(gpu, dims, n, its, want):
     st = gpu

@@ -89,12 +89,13 @@ bool rows_mixed_fused_ok(const Cfg &c);
 st_status launch_rows_defect(const Cfg &c, bool f16, const float *x, const uint16_t *e_old, float *x_out,
                              uint16_t *d, uint16_t *e_first, hipStream_t s);
 
-// 2D streaming temporal blocking, four fused sweeps per pass
+// 2D streaming temporal blocking, four or six fused sweeps per pass
 // (sweep_stream2d.hip); `single` is the per-sweep configuration for the
 // sweeps outside fused passes.
 bool stream2d_eligible(const Cfg &c);
 st_status resolve_stream2d(Cfg &c);
 st_status run_stream2d(const Cfg &c, const Cfg &single, void *u1, void *u2, hipStream_t s, int *in_u2);
+int stream2d_launches(const Cfg &c);  // launches with equal shells
 
 // Streaming temporal blocking, whole rows per wave (sweep_stream.hip).
 bool stream_eligible(const Cfg &c);

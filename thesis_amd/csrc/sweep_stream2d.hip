@@ -3,7 +3,7 @@
 // reference evaluates 2D up to 32768^2, scripts/evaluate_stencils.py:71-76).
 //
 // Each wave owns a segment of 512 consecutive columns (8 per lane) and
-// marches it along y over a chunk of rows, carrying T time levels at once:
+// marches it along y over a chunk of rows, carrying T (4 or 6) time levels:
 // when row m of level 0 arrives, level l completes row m - l from the three
 // rows m-l-1 .. m-l+1 of level l-1 and passes it on, so one HBM pass reads
 // the field once and writes it once for T sweeps.  Per level and row a lane
@@ -181,11 +181,45 @@ __global__ void k_flag_set(int *f) { if (threadIdx.x == 0) *f = 1; }
 
 // ---------------------------------------------------------------- host side
 
-constexpr int kS2Steps = 4;  // sweeps fused per pass
-
 bool stream2d_eligible(const Cfg &c)
 {
     return c.dims == 2 && c.dtype == ST_F32 && c.r == 1 && c.nx % 8 == 0 && c.nx >= 16 && c.ny >= 3;
+}
+
+// sweeps per pass of the deep kernel (64 sweeps at 32768^2: 6 per pass
+// 2891 GCells/s, 8 per pass 2697 with 256 VGPRs and one wave per SIMD, 4
+// per pass 2155)
+constexpr int kS2Hi = 6;
+
+// Fused passes of kS2Hi and of 4 sweeps: the most sweeps fused, then the
+// fewest passes, with an even number of passes in all (each pass swaps the
+// buffers, and a fused pass advances an even number of sweeps, so an even
+// pass count leaves the state where the same number of single sweeps
+// would); deep passes only when c.tsteps asks for them.
+static void stream2d_schedule(const Cfg &c, int *a8, int *b4)
+{
+    *a8 = 0;
+    *b4 = 2 * (c.iters / 8);
+    if (c.tsteps != kS2Hi) return;
+    int best = 4 * *b4;
+    for (int a = 1; kS2Hi * a <= c.iters; ++a) {
+        int b = (c.iters - kS2Hi * a) / 4;
+        if ((a + b) & 1) --b;
+        if (b < 0) continue;
+        const int fused = kS2Hi * a + 4 * b;
+        if (fused > best || (fused == best && a > *a8)) {
+            best = fused;
+            *a8 = a;
+            *b4 = b;
+        }
+    }
+}
+
+int stream2d_launches(const Cfg &c)
+{
+    int a8, b4;
+    stream2d_schedule(c, &a8, &b4);
+    return a8 + b4 + (c.iters - kS2Hi * a8 - 4 * b4);
 }
 
 st_status resolve_stream2d(Cfg &c)
@@ -193,19 +227,26 @@ st_status resolve_stream2d(Cfg &c)
     if (!stream2d_eligible(c))
         return fail(ST_EUNSUPPORTED, "2D streaming form needs FP32, radius 1, nx a multiple of 8 (>= 16) and "
                                      "ny >= 3");
+    if (c.tsteps != 0 && c.tsteps != 4 && c.tsteps != kS2Hi)
+        return fail(ST_EINVAL, "2D streaming form: temporal_steps must be 0, 4 or %d", kS2Hi);
+    const int nseg = (c.nx + kS2Core - 1) / kS2Core;
+    // Rows per chunk: block_y with HEURISTIC=0, else enough chunks for about
+    // 4096 waves, 32 to 256 rows (chunks overlap by 2T rows).  Measured at
+    // 64 sweeps: 8192^2 1266 GCells/s with 256-row chunks, 2064 with 34.
+    int chunk = (int)std::min<long long>(256, std::max<long long>(32, (long long)nseg * c.ny / 4096));
+    if (!c.heuristic && c.by > 0) chunk = c.by;
+    if (chunk > c.ny) chunk = c.ny;
+    // Deep passes by default (64 sweeps, passes of 6 against passes of 4:
+    // 32768^2 2947 against 2155 GCells/s, 16384^2 2832 / 2143, 8192^2
+    // 2475 / 2056, 4096^2 1491 / 1391); temporal_steps 4 keeps passes of 4.
+    if (c.tsteps == 0) c.tsteps = kS2Hi;
     c.kind = 6;
-    c.tsteps = kS2Steps;
     c.cx = 8;
     c.cy = 1;
     c.bx = kS2Cols;
-    c.by = 1;
+    c.by = chunk;
     c.bz = 1;
-    // rows per chunk: block_y with HEURISTIC=0, else 256 (chunks overlap by
-    // 2T rows; 256 keeps that at 3% while giving enough waves)
-    int chunk = (!c.heuristic && c.by > 0) ? c.by : 256;
-    if (chunk > c.ny) chunk = c.ny;
     c.zchunk = chunk;
-    const int nseg = (c.nx + kS2Core - 1) / kS2Core;
     const int nch = (c.ny + chunk - 1) / chunk;
     c.gx = (nseg * nch + 3) / 4;
     c.gy = c.gz = 1;
@@ -213,24 +254,26 @@ st_status resolve_stream2d(Cfg &c)
     return ST_OK;
 }
 
-static const void *stream2d_fn(int op)
+static const void *stream2d_fn(int op, int t)
 {
-    return op == OP_THESIS ? reinterpret_cast<const void *>(&k_stream2d<OP_THESIS, kS2Steps>)
-                           : reinterpret_cast<const void *>(&k_stream2d<OP_JACOBI, kS2Steps>);
+    if (t == kS2Hi)
+        return op == OP_THESIS ? reinterpret_cast<const void *>(&k_stream2d<OP_THESIS, kS2Hi>)
+                               : reinterpret_cast<const void *>(&k_stream2d<OP_JACOBI, kS2Hi>);
+    return op == OP_THESIS ? reinterpret_cast<const void *>(&k_stream2d<OP_THESIS, 4>)
+                           : reinterpret_cast<const void *>(&k_stream2d<OP_JACOBI, 4>);
 }
 
-// Passes: n4 of kS2Steps fused sweeps (an even number of passes, so the
-// state after them is in u1 as after the same number of single sweeps),
-// then the remaining iters - 4*n4 sweeps one at a time with the per-sweep
-// kernel of `single`; with different shells, or when the shells cannot be
-// compared on the host (graph capture, the no-alloc entry point), every
-// sweep runs singly.
+// Fused passes as stream2d_schedule plans them, then the remaining sweeps one
+// at a time with the per-sweep kernel of `single`; with different shells,
+// or when the shells cannot be compared on the host (graph capture, the
+// no-alloc entry point), every sweep runs singly.
 st_status run_stream2d(const Cfg &c, const Cfg &single, void *u1, void *u2, hipStream_t s, int *in_u2)
 {
     *in_u2 = c.iters & 1;
     if (c.iters <= 0) return ST_OK;
-    int n4 = 2 * (c.iters / (2 * kS2Steps));
-    if (n4 > 0) {
+    int a8 = 0, b4 = 0;
+    stream2d_schedule(c, &a8, &b4);
+    if (a8 + b4 > 0) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         bool same = false;
         if (!workspace_is_fixed() && hipStreamIsCapturing(s, &cs) == hipSuccess &&
@@ -250,7 +293,7 @@ st_status run_stream2d(const Cfg &c, const Cfg &single, void *u1, void *u2, hipS
             if (e != hipSuccess) return hip_check(e, "2D shell comparison read-back");
             same = h != 0;
         }
-        if (!same) n4 = 0;
+        if (!same) a8 = b4 = 0;
     }
     S2Args a;
     a.nx = c.nx;
@@ -260,15 +303,15 @@ st_status run_stream2d(const Cfg &c, const Cfg &single, void *u1, void *u2, hipS
     a.nchunks = (c.ny + c.zchunk - 1) / c.zchunk;
     a.nwaves = a.nseg * a.nchunks;
     const float kf = c.op == OP_THESIS ? 4.0f : (float)c.weight;
-    const void *fn = stream2d_fn(c.op);
-    for (int p = 0; p < n4; ++p) {
+    for (int p = 0; p < a8 + b4; ++p) {
         const float *x = static_cast<const float *>((p & 1) ? u2 : u1);
         float *y = static_cast<float *>((p & 1) ? u1 : u2);
         void *args[] = {&x, &y, &a, const_cast<float *>(&kf)};
-        hipError_t e = hipLaunchKernel(fn, dim3((a.nwaves + 3) / 4), dim3(256), args, 0, s);
+        hipError_t e = hipLaunchKernel(stream2d_fn(c.op, p < a8 ? kS2Hi : 4), dim3((a.nwaves + 3) / 4), dim3(256),
+                                       args, 0, s);
         if (e != hipSuccess) return hip_check(e, "2D stream pass launch");
     }
-    for (int i = kS2Steps * n4; i < c.iters; ++i) {
+    for (int i = kS2Hi * a8 + 4 * b4; i < c.iters; ++i) {
         const void *in = (i & 1) ? u2 : u1;
         void *out = (i & 1) ? u1 : u2;
         st_status st = launch_march(single, in, out, s);
