@@ -50,6 +50,7 @@ struct BatchProblem {
     int tyf, tiles_yf, item0f;
     int zchunkf;    // planes per fused tile
     int n2;
+    int ncf, nlf;   // fused form: level-1 cells and level-0 load slots per thread
     float *u1, *u2;
 };
 
@@ -308,10 +309,17 @@ __device__ __forceinline__ void bp_tile(const BatchProblem &q, int local, int sw
 // even passes and u2 on odd ones and writes the other; with an even number
 // of fused passes per problem the state after them is in u1, as after the
 // same number of single sweeps.
-// cells per thread and level: 8 at R = 1, 4 at R = 2 (registers)
+//
+// Level-1 cells cover the frame's rows plus the halo rows inside the
+// domain.  A problem whose whole y extent is one frame takes the smallest
+// power of two of cells (and load slots) per thread that covers its plane,
+// so small planes do not pay for the largest frame's registers and LDS
+// traffic.
+// cells per thread and level at most: 8 at R = 1, 4 at R = 2 (registers)
 __host__ __device__ constexpr int bp2_cells(int r) { return r == 1 ? 8 : 4; }
-constexpr int kBp2L0 = 6144;        // LDS floats of the level-0 window: <= 4096 loaded + slot padding +
-                                    // reads of cells past the frame (< 4096 + 2R*nx + R <= 5918)
+constexpr int kBp2L0 = 6144;        // LDS floats of the level-0 window: <= 4096 loaded + slot padding;
+                                    // reads of cells past the frame may run on into the level-1 plane
+constexpr int kBp2Dummy = kBp2L0 - kBpThreads;  // store target of the unused load slots (past 4096 + 511)
 constexpr int kBp2L1 = 7168;        // level-1 plane: level-2 reads reach 3R*nx + 4095 + R <= 6827
 constexpr int kBp2Lds = kBp2L0 + kBp2L1;
 
@@ -326,17 +334,27 @@ __host__ __device__ inline int bp2_rows(int nx, int ny, int r)
     return f < 1 ? 0 : (f < ny ? f : ny);
 }
 
-template <int OP, int R>
+// NC level-1 cells per thread, NL level-0 load slots per thread.
+template <int OP, int R, int NC, int NL>
 __device__ __forceinline__ void bp_tile2(const BatchProblem &q, int local, int pass, float *lds)
 {
-    constexpr int NC = bp2_cells(R);
+    static_assert(NC <= bp2_cells(R) && NL <= kBpLoads, "fused tile shape");
     const int t = threadIdx.x;
     const int nx = q.g.nx, ny = q.g.ny, nz = q.g.nz;
     const long long sz = q.g.sz;
     const int ty = local % q.tiles_yf, tz = local / q.tiles_yf;
     const int y0 = ty * q.tyf;
     const int fr = min(q.tyf, ny - y0);
-    const int n1c = (fr + 2 * R) * nx;  // level-1 cells: rows y0-R .. y0+fr+R-1
+    // level-1 cells: rows l1lo .. l1hi-1, starting c0 cells into the LDS
+    // layout whose row 0 is domain row y0 - R.  The smaller shapes (whole-y
+    // frames) keep to the rows inside the domain, as their cell count is
+    // chosen by; the largest shape takes the frame's rows and R halo rows on
+    // each side as they are (one register fewer in its address arithmetic).
+    constexpr bool kClamp = NC < bp2_cells(R);
+    const int l1lo = kClamp ? max(y0 - R, 0) : y0 - R;
+    const int l1hi = kClamp ? min(y0 + fr + R, ny) : y0 + fr + R;
+    const int c0 = kClamp ? (l1lo - (y0 - R)) * nx : 0;
+    const int n1c = (l1hi - l1lo) * nx;
     const int n2c = fr * nx;            // level-2 cells: rows y0 .. y0+fr-1
     const int z0 = tz * q.zchunkf, z1 = min(z0 + q.zchunkf, nz);
     const float *in = (pass & 1) ? q.u2 : q.u1;
@@ -355,7 +373,7 @@ __device__ __forceinline__ void bp_tile2(const BatchProblem &q, int local, int p
         const int c = t + i * kBpThreads;
         if (c < n1c) {
             const int r = c / nx, x = c - r * nx;
-            const int y = y0 - R + r;
+            const int y = l1lo + r;
             if (x >= R && x < nx - R && y >= R && y < ny - R) intr1 |= 1u << i;
         }
         if (c < n2c) {
@@ -366,7 +384,7 @@ __device__ __forceinline__ void bp_tile2(const BatchProblem &q, int local, int p
         }
     }
 
-    float ring1[R][NC], part1[R][NC], ring2[R][NC], part2[R][NC], pf[kBpLoads], pf2[kBpLoads];
+    float ring1[R][NC], part1[R][NC], ring2[R][NC], part2[R][NC], pf[NL], pf2[NL];
 #pragma unroll
     for (int j = 0; j < R; ++j)
 #pragma unroll
@@ -377,24 +395,23 @@ __device__ __forceinline__ void bp_tile2(const BatchProblem &q, int local, int p
         __builtin_amdgcn_make_buffer_rsrc((void *)in, (short)0, (int)(sz * nz * 4), 0x00020000);
     const __amdgpu_buffer_rsrc_t rout =
         __builtin_amdgcn_make_buffer_rsrc((void *)out, (short)0, (int)(sz * nz * 4), 0x00020000);
-    unsigned slot_off[kBpLoads];
+    unsigned slot_off[NL];
 #pragma unroll
-    for (int i = 0; i < kBpLoads; ++i) slot_off[i] = (unsigned)(i >= nslot) << 31;
-    auto load_plane = [&](int k, float (&d)[kBpLoads], bool valid) {
+    for (int i = 0; i < NL; ++i) slot_off[i] = (unsigned)(i >= nslot) << 31;
+    auto load_plane = [&](int k, float (&d)[NL], bool valid) {
         k = min(max(k, 0), nz - 1);
         const unsigned base = ((unsigned)(k * sz + (long long)rlo * nx) * 4u) | ((unsigned)!valid << 31);
 #pragma unroll
-        for (int i = 0; i < kBpLoads; ++i) {
+        for (int i = 0; i < NL; ++i) {
             const unsigned off = (base | slot_off[i]) + (unsigned)(t + i * kBpThreads) * 4u;
             d[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rin, (int)off, 0, 0));
         }
     };
     const int kin_lo = z0 - 2 * R, kin_hi = z1 + 2 * R;
-    auto step = [&](int m, float (&pa)[kBpLoads]) {
+    auto step = [&](int m, float (&pa)[NL]) {
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int i = 0; i < kBpLoads; ++i)
-            L0[(i < nslot ? ldoff + i * kBpThreads : kBpLoads * kBpThreads) + t] = pa[i];
+        for (int i = 0; i < NL; ++i) L0[(i < nslot ? ldoff + i * kBpThreads : kBp2Dummy) + t] = pa[i];
         load_plane(m + 2, pa, m + 2 < kin_hi);
         __syncthreads();
         // level 1: plane m - R completes
@@ -402,7 +419,7 @@ __device__ __forceinline__ void bp_tile2(const BatchProblem &q, int local, int p
         float v1[NC];
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
-            const int b = R * nx + t + i * kBpThreads;
+            const int b = R * nx + c0 + t + i * kBpThreads;
             float s = 0.0f;
 #pragma unroll
             for (int d = R; d >= 1; --d) s += L0[b - d];
@@ -427,7 +444,7 @@ __device__ __forceinline__ void bp_tile2(const BatchProblem &q, int local, int p
             part1[R - 1][i] = s;
         }
 #pragma unroll
-        for (int i = 0; i < NC; ++i) L1[R * nx + t + i * kBpThreads] = v1[i];
+        for (int i = 0; i < NC; ++i) L1[R * nx + c0 + t + i * kBpThreads] = v1[i];
         __syncthreads();
         // level 2: plane m - 2R completes
         const int k = m - 2 * R;
@@ -549,8 +566,18 @@ __global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(OP =
     const BatchProblem &q = P[find_problem(starts, np, tile)];
     if (pass >= q.n2) return;
     const int local = tile - q.item0f;
-    if (q.g.r == 1) bp_tile2<OP, 1>(q, local, pass, lds);
-    else bp_tile2<OP, 2>(q, local, pass, lds);
+    if (q.g.r == 1) {
+        if (q.ncf == 8) bp_tile2<OP, 1, 8, 8>(q, local, pass, lds);
+        else if (q.ncf == 4) bp_tile2<OP, 1, 4, 4>(q, local, pass, lds);
+        else if (q.ncf == 2) bp_tile2<OP, 1, 2, 2>(q, local, pass, lds);
+        else bp_tile2<OP, 1, 1, 1>(q, local, pass, lds);
+    } else {
+        // (no 4-cell, 4-slot shape at R = 2: with it the kernel's register
+        // allocation spills)
+        if (q.ncf == 4) bp_tile2<OP, 2, 4, 8>(q, local, pass, lds);
+        else if (q.ncf == 2) bp_tile2<OP, 2, 2, 2>(q, local, pass, lds);
+        else bp_tile2<OP, 2, 1, 1>(q, local, pass, lds);
+    }
 }
 
 template <int OP>
@@ -614,6 +641,25 @@ static int fused_chunk(const st_problem &p)
     return want > p.nz ? p.nz : want;
 }
 
+// Cells and load slots per thread of the fused form: a problem whose whole
+// y extent is one frame holds its plane (level 1 and the level-0 window are
+// both ny * nx cells) in the smallest power of two per thread below the
+// largest shape that covers it; framed problems, and planes that need it,
+// take the largest shape (bp2_cells(R) cells, kBpLoads slots).
+static void fused_shape(const st_problem &p, int tiles_yf, int &nc, int &nl)
+{
+    const int top = bp2_cells(p.radius);
+    nc = top;
+    nl = kBpLoads;
+    if (tiles_yf != 1) return;
+    const long long plane = (long long)p.nx * p.ny;
+    for (int c = 1; c < top; c *= 2)
+        if (plane <= (long long)c * kBpThreads) {
+            nc = nl = c;
+            return;
+        }
+}
+
 st_status build_batch(const st_problem *probs, int n, const st_tile_rule *rules, int nrules,
                       std::vector<BatchProblem> &out, int &nitems, int &nitemsf_out, int &maxit, bool plane_form)
 {
@@ -660,6 +706,7 @@ st_status build_batch(const st_problem *probs, int n, const st_tile_rule *rules,
         // frames as tall as the level-0 window allows
         b.tyf = plane_form ? bp2_rows(p.nx, p.ny, p.radius) : 0;
         b.tiles_yf = b.tyf > 0 ? (p.ny + b.tyf - 1) / b.tyf : 0;
+        fused_shape(p, b.tiles_yf, b.ncf, b.nlf);
         b.item0f = nitemsf;
         b.zchunkf = fused_chunk(p);
         b.n2 = 0;
