@@ -446,11 +446,16 @@ STREAM_SHAPES = [
     ((128, 70, 61), {"block_z": 17, "heuristic": 0}),  # explicit z chunk
     ((512, 60, 40), {"coarsen_y": 2, "block_y": 8, "heuristic": 0}),   # 2 rows of 8 cells
     ((200, 45, 21), {"coarsen_y": 4, "block_y": 4, "heuristic": 0}),   # 4 rows of 4 cells
-    # over 512 columns: one row of 16 cells per lane
+    # over 512 columns: two waves of 8 cells per row (seam through LDS; the
+    # kernel for equal shells takes up to 6 wave rows, the other 4)
     ((1024, 22, 14), {}),
     ((1000, 17, 9), {}),                      # last quads past the row
-    ((516, 13, 11), {}),                      # second half of the lanes mostly idle
+    ((516, 13, 11), {}),                      # the right wave holds one quad
+    ((1024, 40, 12), {"block_y": 5, "heuristic": 0}),   # 5 wave rows (4 for different shells)
+    ((772, 70, 9), {"block_y": 3, "block_z": 4, "heuristic": 0}),  # 2 core rows, short chunks
+    # one wave of 16 cells per row on request
     ((1024, 40, 12), {"coarsen_y": 1, "block_y": 6, "heuristic": 0}),  # 6 waves, 2 core rows
+    ((1000, 17, 9), {"coarsen_y": 1, "block_y": 8, "heuristic": 0}),
 ]
 
 
@@ -518,6 +523,24 @@ def test_coop_stream_one_shell_cell_differs(gpu, oracle, face, per_pass):
                                   launch_per_pass=per_pass)
         got, _ = run_gpu(st, p, u1, u2)
         assert_bitwise(got, want, f"shell cell {face} its={its} per_pass={per_pass}")
+
+
+def test_coop_stream_two_waves_per_row_frame(gpu):
+    """Over 512 columns the streaming form puts two waves of 8 cells on each
+    row; at 1024^3 the kernel for equal shells takes 6 wave rows (12 rows, 8
+    stored), and one wave of 16 cells stays available through coarsen_y 1."""
+    st = gpu
+    p = st.StencilParams.make(dim=1024, radius=1, iterations=100, op=1, variant="coop", heuristic=1)
+    cfg = st.launch_config(p)
+    assert (cfg["coarsen_x"], cfg["coarsen_y"], cfg["block_y"]) == (8, 2, 8), cfg
+    q = st.StencilParams.make(dim=1024, radius=1, iterations=100, op=1, variant="coop", temporal_form=3,
+                              heuristic=0, block_x=1024, block_y=6, block_z=64, coarsen_y=1)
+    cfg = st.launch_config(q)
+    assert (cfg["coarsen_x"], cfg["coarsen_y"], cfg["block_y"], cfg["zchunk"]) == (16, 1, 2, 64), cfg
+    # 7 wave rows of two waves exceed the 768-thread kernel
+    r = q.copy(coarsen_y=0, block_y=7)
+    with pytest.raises(st.StencilError):
+        st.check_early_exit(r)
 
 
 def test_coop_stream_form_rejects_unsupported(gpu):
@@ -836,7 +859,7 @@ def test_coop_cooperative_launch_timed_on_null_stream(gpu, oracle, form):
     (3, 320, 100, "coop"),           # both fields above 192 MiB
     (3, 288, 100, "smem_register"),  # below: the row kernel
     (3, 512, 2, "smem_register"),    # too few sweeps for a fused pass
-    (3, 1024, 100, "coop"),          # one row of 16 cells per lane
+    (3, 1024, 100, "coop"),          # two waves per row
     (3, 768, 100, "smem_register"),  # a quarter of those lanes idle: the row kernel
     (2, 256, 64, "base"),
     (2, 1024, 64, "base"),           # launch-bound

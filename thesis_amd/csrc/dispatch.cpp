@@ -181,6 +181,7 @@ static st_status resolve(const st_params *p, Cfg &c, bool mixed)
     c.wx = c.wy = 0;
     c.hs = 1;
     c.rowc = 0;
+    c.wy_d = c.core_d = c.zchunk_d = c.lds_d = 0;
     c.shell_eq = nullptr;
     c.zchunk = 0;
     c.lds = 0;
@@ -314,7 +315,7 @@ st_status stencil_launch_config(const st_params *p, st_launch_info *info)
             if (c.kind == 5) n = c.iters;
             info->launches = n ? (c.per_pass ? n : 1) : 0;
             // 2D streaming form, always one launch per pass (with equal
-            // shells: fused passes of 8 and 4 sweeps, then single sweeps)
+            // shells: fused passes of six and four sweeps, then single sweeps)
             if (c.kind == 6) info->launches = stream2d_launches(c);
         } else {
             info->launches = c.iters;

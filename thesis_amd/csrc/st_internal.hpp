@@ -31,6 +31,8 @@ struct Cfg {
     int wx, wy;          // COOP register form: waves across x and y
     int hs;              // SMEM_REGISTER row form: halo strips per wave
     int rowc;            // row form: 1 = consecutive columns per lane (k_rowc)
+    int wy_d, core_d;    // COOP streaming form, kernel for different shells (when its
+    int zchunk_d, lds_d; // frame differs): wave rows, core rows, z chunk, LDS bytes
     const int *shell_eq; // device flag: both buffers' shells hold the same bits
                          // (row form stores whole rows then); null: unknown
 };

@@ -22,13 +22,15 @@
 //     pass moves the same HBM bytes as with equal shells.  Only interior
 //     cells are stored, so both shells stay untouched.
 //
-// Layout: a workgroup of WY waves owns a frame of RPW*WY full rows (a wave
-// holds RPW rows, each lane CX cells of each, in quads of four consecutive
-// columns 256 apart: two rows of 4 or 8 cells up to 512 columns, one row of
-// 16 cells up to 1024) and marches it along z over a chunk of planes.  Per level and plane:
-//   x neighbours   own registers, lane edges by DPP wave shifts (the row
-//                  fits one wave, so the only missing neighbours are the
-//                  ones outside the grid, next to shell cells);
+// Layout: a workgroup of WY wave rows owns a frame of RPW*WY full rows (a
+// wave row is WX waves side by side; each wave holds RPW rows, each lane CX
+// cells of each, in quads of four consecutive columns 256 apart: two rows
+// of 4 or 8 cells up to 512 columns; over 512 columns two waves of 8 cells
+// per row, or one wave of 16 cells on request) and marches it along z over
+// a chunk of planes.  Per level and plane:
+//   x neighbours   own registers, lane edges by DPP wave shifts; with two
+//                  waves per row the seam columns (one value per row and
+//                  wave) go through LDS with the y exchange;
 //   y neighbours   own registers, frame-row edges through one LDS exchange
 //                  of both rows (one barrier);
 //   z neighbours   a running partial sum: when plane k arrives its x, y
@@ -251,14 +253,19 @@ __device__ __forceinline__ void sm_take(SmPlane<CX, RPW> &d, const SmPlane<CX, R
 }
 
 // Partial sums of the plane `nw` (x, y, then the z-previous plane `zm`) in
-// the reference's order; the y edges come from the neighbouring waves
-// through `reg` (one LDS region of 2*WY*CX/4*64 float4).
-template <int CX, int RPW>
+// the reference's order; the y edges come from the neighbouring wave rows
+// through `reg` (one LDS region of 2*waves*CX/4*64 float4).  With two waves
+// per row (WX = 2, RPW = 2: both rows of a wave are edge rows, so the
+// region holds them whole) the seam column next to a wave's first or last
+// column is read from the partner wave's rows there.  w: wave index in the
+// workgroup, wy: wave rows.
+template <int CX, int RPW, int WX>
 __device__ __forceinline__ void sm_partial(SmPlane<CX, RPW> &out, const SmPlane<CX, RPW> &nw,
-                                           const SmPlane<CX, RPW> &zm, float4 *reg, int w, int wy,
-                                           int lane)
+                                           const SmPlane<CX, RPW> &zm, float4 *reg, int w, int wy, int lane)
 {
+    static_assert(WX == 1 || (WX == 2 && RPW == 2), "one wave per row, or two with two rows each");
     constexpr int H = CX / 4;
+    const int wr = w / WX, wx = w % WX;
     // edge rows only: row 0 (e = 0) and row RPW-1 (e = 1)
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
@@ -269,8 +276,27 @@ __device__ __forceinline__ void sm_partial(SmPlane<CX, RPW> &out, const SmPlane<
                 make_float4(nw.v[r][4 * h], nw.v[r][4 * h + 1], nw.v[r][4 * h + 2], nw.v[r][4 * h + 3]);
     }
     __syncthreads();
-    const int wm = w == 0 ? wy - 1 : w - 1;   // wrapped rows are outside the
-    const int wp = w == wy - 1 ? 0 : w + 1;   // frame core and never stored
+    // wrapped rows are outside the frame core and never stored
+    const int wm = (wr == 0 ? wy - 1 : wr - 1) * WX + wx;
+    const int wp = (wr == wy - 1 ? 0 : wr + 1) * WX + wx;
+    // the partner wave's seam cell of row r (edge row e = r): the left
+    // half's last column (quad H-1, lane 63, .w) or the right half's first
+    // (quad 0, lane 0, .x)
+    // (one value per row for the whole wave, read first and held in scalar
+    // registers, so that it holds no vector register while the y edges are
+    // read)
+    float sc[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) sc[r] = 0.0f;
+    if constexpr (WX == 2) {
+        const float *seam = reinterpret_cast<const float *>(reg) +
+                            4 * (wx == 1 ? ((w - 1) * 2 * H + H - 1) * 64 + 63 : (w + 1) * 2 * H * 64) +
+                            (wx == 1 ? 3 : 0);
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+            sc[r] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(seam[r * H * 64 * 4])));
+        __builtin_amdgcn_sched_barrier(0);
+    }
     float ym[CX], yp[CX];
 #pragma unroll
     for (int h = 0; h < H; ++h) {
@@ -309,10 +335,14 @@ __device__ __forceinline__ void sm_partial(SmPlane<CX, RPW> &out, const SmPlane<
         float xa[CX], xb[CX];
 #pragma unroll
         for (int h = 0; h < H; ++h) {
-            const float lft = h > 0 ? pa[h - 1] : pa[0];
-            const float rgt = h < H - 1 ? pb[h + 1] : pb[H - 1];
-            xa[4 * h] = first ? lft : pa[h];
-            xb[4 * h + 3] = last ? rgt : pb[h];
+            if (h == 0)  // lane 0: outside the grid, or the seam (right half)
+                xa[0] = (WX == 2 && first && wx == 1) ? sc[r] : pa[0];
+            else
+                xa[4 * h] = first ? pa[h - 1] : pa[h];
+            if (h == H - 1)  // lane 63: outside the grid, or the seam (left half)
+                xb[CX - 1] = (WX == 2 && last && wx == 0) ? sc[r] : pb[H - 1];
+            else
+                xb[4 * h + 3] = last ? pb[h + 1] : pb[h];
 #pragma unroll
             for (int j = 1; j < 4; ++j) {
                 xa[4 * h + j] = nw.v[r][4 * h + j - 1];
@@ -424,20 +454,24 @@ __device__ __forceinline__ void sm_store(float *__restrict__ Y, const SmPlane<CX
 // One pass of T (1 or 2) sweeps X -> Y over all tiles of this workgroup.
 // DIFF (T = 2, the buffers' shells differ): level-0 shell values from S0
 // (u1), level-1 shell values from S1 (u2).
-template <int OP, int CX, int RPW, int T, bool DIFF>
+template <int OP, int CX, int RPW, int T, bool DIFF, int WX>
 __device__ __forceinline__ void sm_pass(const float *__restrict__ X, float *__restrict__ Y,
                                         const float *__restrict__ S0, const float *__restrict__ S1,
                                         const float *__restrict__ xs, const SmArgs &a, float kc, float4 *lds,
                                         unsigned bid, unsigned nblk, bool same)
 {
     static_assert(!DIFF || T == 2, "single-sweep passes read the shell of the buffer they stream");
+    constexpr int H = CX / 4;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wy = blockDim.x >> 6;
+    const int nwv = blockDim.x >> 6;  // waves
+    const int wy = nwv / WX;          // wave rows
+    const int wr = w / WX;            // this wave's row of waves
+    const int wx = w % WX;            // and its half of the row
     SmLane<CX> l;
 #pragma unroll
-    for (int h = 0; h < CX / 4; ++h) {
-        const int x = 256 * h + 4 * lane;
+    for (int h = 0; h < H; ++h) {
+        const int x = 256 * (wx * H + h) + 4 * lane;
         l.act[h] = x < a.nx;
         l.xo[h] = l.act[h] ? x : a.nx - 4;  // quads past the row reload its last cells
         l.lo[h] = x == 0;
@@ -445,9 +479,9 @@ __device__ __forceinline__ void sm_pass(const float *__restrict__ X, float *__re
         l.whole[h] = l.act[h] && (same || (!l.lo[h] && !l.hi[h]));
     }
     float4 *reg0 = lds;
-    float4 *reg1 = lds + 2 * wy * (CX / 4) * 64;
+    float4 *reg1 = lds + 2 * nwv * H * 64;
     SmXSlice xsl;
-    xsl.v = reinterpret_cast<float2 *>(lds + 4 * wy * (CX / 4) * 64);
+    xsl.v = reinterpret_cast<float2 *>(lds + 4 * nwv * H * 64);
     xsl.rows = wy * RPW;
     xsl.stride = a.zchunk + 5;
 
@@ -458,7 +492,7 @@ __device__ __forceinline__ void sm_pass(const float *__restrict__ X, float *__re
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
             // wave-uniform row constants, kept in scalar registers
-            const int fr = w * RPW + r;  // frame row
+            const int fr = wr * RPW + r;  // frame row
             const int gy = __builtin_amdgcn_readfirstlane(fy0 + fr);
             const int gyc = min(max(gy, 0), a.ny - 1);
             t.rowoff[r] = gyc * a.nx;
@@ -504,18 +538,18 @@ __device__ __forceinline__ void sm_pass(const float *__restrict__ X, float *__re
         if constexpr (DIFF) sm_load_sh1<CX, RPW>(sh1, S1, a, t, l, kin_lo - 1);
         for (int kin = kin_lo; kin < kin_hi; ++kin) {
             SmPlane<CX, RPW> nw0;
-            sm_take<CX, RPW, DIFF>(nw0, pf, xsl, w, kin, l);
+            sm_take<CX, RPW, DIFF>(nw0, pf, xsl, wr, kin, l);
             if (kin + 1 < kin_hi) sm_load<CX, RPW, DIFF>(pf, X, S0, a, t, l, kin + 1);
             // level 1, plane kin-1
             const int k1 = kin - 1;
             SmPlane<CX, RPW> o1;
-            sm_finish<OP, CX, RPW, DIFF>(o1, P1, nw0, c0, kc, k1 <= 0 || k1 >= a.nz - 1, t, l, sh1, xsl, w, S1, a,
-                                         k1);
+            sm_finish<OP, CX, RPW, DIFF>(o1, P1, nw0, c0, kc, k1 <= 0 || k1 >= a.nz - 1, t, l, sh1, xsl, wr, S1,
+                                         a, k1);
             // the level-1 y-shell rows the next step needs (plane kin)
             if constexpr (DIFF) sm_load_sh1<CX, RPW>(sh1, S1, a, t, l, kin);
             // one sweep: alternate the two regions so that a plane's LDS
             // writes never meet the previous plane's reads
-            sm_partial<CX, RPW>(P1, nw0, c0, (T == 1 && (kin & 1)) ? reg1 : reg0, w, wy, lane);
+            sm_partial<CX, RPW, WX>(P1, nw0, c0, (T == 1 && (kin & 1)) ? reg1 : reg0, w, wy, lane);
             c0 = nw0;
             if constexpr (T == 1) {
                 sm_store<CX, RPW>(Y, o1, a, t, l, k1);
@@ -523,9 +557,9 @@ __device__ __forceinline__ void sm_pass(const float *__restrict__ X, float *__re
                 // level 2, plane kin-2
                 const int k2 = kin - 2;
                 SmPlane<CX, RPW> o2;
-                sm_finish<OP, CX, RPW, false>(o2, P2, o1, c1, kc, k2 <= 0 || k2 >= a.nz - 1, t, l, sh1, xsl, w,
+                sm_finish<OP, CX, RPW, false>(o2, P2, o1, c1, kc, k2 <= 0 || k2 >= a.nz - 1, t, l, sh1, xsl, wr,
                                               S1, a, k2);
-                sm_partial<CX, RPW>(P2, o1, c1, reg1, w, wy, lane);
+                sm_partial<CX, RPW, WX>(P2, o1, c1, reg1, w, wy, lane);
                 c1 = o1;
                 sm_store<CX, RPW>(Y, o2, a, t, l, k2);
             }
@@ -552,7 +586,7 @@ constexpr int sm_max_threads()
 // read the comparison flags and the one whose case does not hold returns at
 // once.  Single-sweep passes read the shell of the buffer they stream, which
 // is right for either case, and run in whichever kernel stays.
-template <int OP, int CX, int RPW, bool DIFF>
+template <int OP, int CX, int RPW, bool DIFF, int WX>
 __global__ __launch_bounds__((sm_max_threads<CX, RPW, DIFF>())) void k_stream(float *__restrict__ u1,
                                                                             float *__restrict__ u2, SmArgs a,
                                                                             float kc)
@@ -568,25 +602,27 @@ __global__ __launch_bounds__((sm_max_threads<CX, RPW, DIFF>())) void k_stream(fl
     for (int p = a.p0; p < npass; ++p) {
         const float *X = (p & 1) ? u2 : u1;
         float *Y = (p & 1) ? u1 : u2;
-        if (p < a.n2) sm_pass<OP, CX, RPW, 2, DIFF>(X, Y, u1, u2, a.xs, a, kc, lds, bid, nblk, same);
-        else sm_pass<OP, CX, RPW, 1, false>(X, Y, u1, u2, a.xs, a, kc, lds, bid, nblk, same);
+        if (p < a.n2) sm_pass<OP, CX, RPW, 2, DIFF, WX>(X, Y, u1, u2, a.xs, a, kc, lds, bid, nblk, same);
+        else sm_pass<OP, CX, RPW, 1, false, WX>(X, Y, u1, u2, a.xs, a, kc, lds, bid, nblk, same);
         if (p + 1 < npass) cooperative_groups::this_grid().sync();
     }
 }
 
 // ---------------------------------------------------------------- host side
 
-// Cells per lane: 4 up to 256 columns, 8 up to 512, 16 up to 1024 (one row
-// per wave then).
+// Cells per lane: 4 up to 256 columns, 8 up to 512 (one wave per row) and
+// up to 1024 (two waves per row; coarsen_y = 1 asks for one wave of 16
+// cells per lane instead).
 static int stream_cx(const Cfg &c)
 {
     if (c.nx % 4 || c.nx > 1024) return 0;
-    return c.nx <= 256 ? 4 : c.nx <= 512 ? 8 : 16;
+    return c.nx <= 256 ? 4 : 8;
 }
 
-// AUTO's use of the form: one row of 16 cells per lane leaves a quarter of
-// the lanes idle at 768 columns, where the row kernel measured faster (768^3:
-// 647 against 601 GCells/s); at 1024^3 the form gives 1010 against 697.
+// AUTO's use of the form: over 512 columns a quarter of the lanes idle at
+// 768 columns, where the row kernel measured faster (768^3, one wave of 16
+// cells per row: 647 against 601 GCells/s); at 1024^3 the form gives 1010
+// against 697 with that layout.
 bool stream_auto_ok(const Cfg &c) { return stream_eligible(c) && (c.nx <= 512 || c.nx > 896); }
 
 bool stream_eligible(const Cfg &c)
@@ -595,18 +631,20 @@ bool stream_eligible(const Cfg &c)
            c.nx >= 8 && c.ny >= 3 && c.nz >= 3 && stream_cx(c) != 0;
 }
 
-template <int CX, int RPW, bool DIFF>
+template <int CX, int RPW, bool DIFF, int WX = 1>
 static const void *stream_fn_t(int op)
 {
-    return op == OP_THESIS ? reinterpret_cast<const void *>(&k_stream<OP_THESIS, CX, RPW, DIFF>)
-                           : reinterpret_cast<const void *>(&k_stream<OP_JACOBI, CX, RPW, DIFF>);
+    return op == OP_THESIS ? reinterpret_cast<const void *>(&k_stream<OP_THESIS, CX, RPW, DIFF, WX>)
+                           : reinterpret_cast<const void *>(&k_stream<OP_JACOBI, CX, RPW, DIFF, WX>);
 }
 
-// Instantiated shapes: 16 cells x 1 row (nx up to 1024), 8 cells x 2 rows
-// (nx up to 512), 4 cells x 2 or 4 rows (nx up to 256).  8 x 4 needs more
-// than 256 VGPRs and is not built.
-static const void *stream_fn(int op, int cx, int rpw, bool diff)
+// Instantiated shapes: 8 cells x 2 rows on two waves per row (nx up to
+// 1024), 16 cells x 1 row (nx up to 1024), 8 cells x 2 rows (nx up to 512),
+// 4 cells x 2 or 4 rows (nx up to 256).  8 x 4 needs more than 256 VGPRs
+// and is not built.
+static const void *stream_fn(int op, int cx, int rpw, bool diff, int wx)
 {
+    if (wx == 2) return diff ? stream_fn_t<8, 2, true, 2>(op) : stream_fn_t<8, 2, false, 2>(op);
     if (cx == 16) return diff ? stream_fn_t<16, 1, true>(op) : stream_fn_t<16, 1, false>(op);
     if (cx == 8) return diff ? stream_fn_t<8, 2, true>(op) : stream_fn_t<8, 2, false>(op);
     if (rpw == 4) return diff ? stream_fn_t<4, 4, true>(op) : stream_fn_t<4, 4, false>(op);
@@ -622,18 +660,19 @@ static int stream_max_waves(int cx, int rpw)
     return std::min(sm_max_threads<4, 2, false>(), sm_max_threads<4, 2, true>()) / 64;
 }
 
-// LDS of a workgroup: the two exchange regions (2*wy edge rows each, in
-// float4 of CX/4 quads per lane), then the tile's x-shell slice (2 buffers
-// x frame rows x (chunk + 5) planes, float2 each).
-static int stream_lds(int wy, int rpw, int cx, int chunk)
+// LDS of a workgroup of wy wave rows of wx waves: the two exchange regions
+// (2 edge rows per wave, in float4 of CX/4 quads per lane), then the
+// tile's x-shell slice (2 buffers x frame rows x (chunk + 5) planes, float2
+// each).
+static int stream_lds(int wy, int rpw, int cx, int chunk, int wx)
 {
-    return 2 * (2 * wy * (cx / 4) * 64 * 16) + 2 * wy * rpw * (chunk + 5) * 8;
+    return 2 * (2 * wy * wx * (cx / 4) * 64 * 16) + 2 * wy * rpw * (chunk + 5) * 8;
 }
 
 // Deepest z chunk whose slice still fits the LDS.
-static int stream_max_chunk(int wy, int rpw, int cx)
+static int stream_max_chunk(int wy, int rpw, int cx, int wx)
 {
-    return (kMaxLds - stream_lds(wy, rpw, cx, -5)) / (2 * wy * rpw * 8) - 5;
+    return (kMaxLds - stream_lds(wy, rpw, cx, -5, wx)) / (2 * wy * rpw * 8) - 5;
 }
 
 // Tiles of one pass for `wy` waves of `rpw` rows per frame: y frames times
@@ -644,7 +683,7 @@ struct SmPlan {
     double cost;
 };
 
-static SmPlan stream_plan(const Cfg &c, int wy, int rpw, int cx, int cus, int chunk_req)
+static SmPlan stream_plan(const Cfg &c, int wy, int rpw, int cx, int wx, int cus, int chunk_req)
 {
     SmPlan p;
     p.wy = wy;
@@ -657,7 +696,7 @@ static SmPlan stream_plan(const Cfg &c, int wy, int rpw, int cx, int cus, int ch
         chunk = (c.nz + chunks - 1) / chunks;
         if (chunk < 4) chunk = 4;
         if (chunk > c.nz) chunk = c.nz;
-        const int cmax = stream_max_chunk(wy, rpw, cx);
+        const int cmax = stream_max_chunk(wy, rpw, cx, wx);
         if (chunk > cmax) {
             const int n = (c.nz + cmax - 1) / cmax;
             chunk = (c.nz + n - 1) / n;
@@ -672,93 +711,125 @@ static SmPlan stream_plan(const Cfg &c, int wy, int rpw, int cx, int cus, int ch
     return p;
 }
 
+// Frame and chunk of one kernel of the form (at most wmax wave rows).
+static st_status stream_choose(const Cfg &c, int rpw, int cx, int wx, int wmax, int cus, SmPlan &best)
+{
+    const int chunk_req = c.zchunk > 0 ? c.zchunk : 0;
+    const int wmin = (4 + rpw) / rpw;  // at least one core row
+    if (!c.heuristic && c.by > 0) {
+        // block_y = wave rows per frame (frame of rpw*block_y rows)
+        if (c.by < wmin || c.by > wmax)
+            return fail(ST_EINVAL, "streaming form: block_y (wave rows per frame) must be %d..%d", wmin, wmax);
+        best = stream_plan(c, c.by, rpw, cx, wx, cus, chunk_req);
+        return ST_OK;
+    }
+    // whole grid in one frame if it fits, else the frame height whose
+    // tiles fill the CUs best
+    const int wfit = (c.ny + 4 + rpw - 1) / rpw;
+    best = stream_plan(c, wfit < wmax ? (wfit < wmin ? wmin : wfit) : wmax, rpw, cx, wx, cus, chunk_req);
+    // candidate frame heights: down to two thirds of the largest, and
+    // not below 8 waves where 8 fit (fewer waves lose the latency
+    // hiding; 512^3: 6 waves x 3 chunks ran 21% slower than 8 x 5)
+    const int wlo = std::max(std::max(wmin, (wmax * 2) / 3), std::min(wmax, 8));
+    if (wfit > wmax)
+        for (int wy = wmax - 1; wy >= wlo; --wy) {
+            const SmPlan q = stream_plan(c, wy, rpw, cx, wx, cus, chunk_req);
+            if (q.cost < best.cost * 0.97) best = q;  // fewer waves must pay for themselves
+        }
+    // Fields of 2.5 times the 256 MiB last-level cache and more: the
+    // plan with the fewest planes marched per plane stored ((chunk + 4)
+    // / chunk) wins as long as 75-92% of the CUs hold a tile (frames of
+    // 2/3 of the largest height or more, as measured; ties go to more
+    // tiles).  At 512^3: 8 waves x 5 chunks of 103 planes (215 tiles)
+    // 9.67 ms per 100 sweeps, 10 x 8 chunks of 64 (256 tiles, the
+    // busiest-CU model's choice) 10.2, 12 x 8 of 64 (208) 9.88, with the
+    // same HBM bytes; at 448^3 8 x 6 of 75 (228 tiles) 6.72 against 6.98.
+    // At 384^3 the busiest-CU model's 256 tiles stay the fastest.
+    if (chunk_req <= 0 && 2LL * c.nx * c.ny * c.nz * 4 >= (640LL << 20)) {
+        SmPlan pick{};
+        double pick_amp = 0;
+        for (int wy = wmax; wy >= wlo; --wy) {
+            const int core_h = wy * rpw - 4;
+            const int tiles_y = (c.ny + core_h - 1) / core_h;
+            for (int nch = 1; nch * tiles_y <= cus; ++nch) {
+                const int chunk = (c.nz + nch - 1) / nch;
+                const int ntiles = tiles_y * ((c.nz + chunk - 1) / chunk);
+                if (chunk < 4 || chunk > stream_max_chunk(wy, rpw, cx, wx) || ntiles * 100 < cus * 75 ||
+                    ntiles * 100 > cus * 92)
+                    continue;
+                const double amp = (chunk + 4.0) / chunk;
+                if (pick.wy == 0 || amp < pick_amp - 1e-9 || (amp < pick_amp + 1e-9 && ntiles > pick.ntiles)) {
+                    pick = stream_plan(c, wy, rpw, cx, wx, cus, chunk);
+                    pick_amp = amp;
+                }
+            }
+        }
+        if (pick.wy) best = pick;
+    }
+    const int lds = stream_lds(best.wy, rpw, cx, best.chunk, wx);
+    if (lds > kMaxLds)
+        return fail(ST_ELDS, "streaming form: a z chunk of %d planes needs %d bytes of LDS (at most %d "
+                             "planes with this frame)", best.chunk, lds, stream_max_chunk(best.wy, rpw, cx, wx));
+    return ST_OK;
+}
+
 st_status resolve_stream(Cfg &c)
 {
     if (!stream_eligible(c))
         return fail(ST_EUNSUPPORTED, "streaming temporal form needs FP32, 3D, radius 1, nx a multiple "
                                      "of 4 up to 1024");
-    const int cx = stream_cx(c);
+    int cx = stream_cx(c);
     // rows per wave: 2, or coarsen_y = 4 with 4 cells per lane (nx <= 256);
-    // one row of 16 cells per lane over 512 columns
-    int rpw = cx == 16 ? 1 : 2;
+    // over 512 columns two waves per row, or coarsen_y = 1: one row of 16
+    // cells per lane
+    int rpw = 2, wx = c.nx > 512 ? 2 : 1;
     if (!c.heuristic && c.cy > 0) {
-        if (cx == 16 ? c.cy != 1 : (c.cy != 2 && !(c.cy == 4 && cx == 4)))
+        if (c.cy != 2 && !(c.cy == 4 && cx == 4) && !(c.cy == 1 && c.nx > 512))
             return fail(ST_EINVAL, "streaming form: coarsen_y must be 2 (or 4 when nx <= 256; 1 when "
                                    "nx > 512)");
         rpw = c.cy;
+        if (rpw == 1) {
+            cx = 16;
+            wx = 1;
+        }
     }
-    const int wmax = stream_max_waves(cx, rpw);
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
         cus = 256;
-    const int chunk_req = c.zchunk > 0 ? c.zchunk : 0;
-    const int wmin = (4 + rpw) / rpw;  // at least one core row
-    SmPlan best;
-    if (!c.heuristic && c.by > 0) {
-        // block_y = waves per frame (frame of rpw*block_y rows)
-        if (c.by < wmin || c.by > wmax)
-            return fail(ST_EINVAL, "streaming form: block_y (waves per frame) must be %d..%d", wmin, wmax);
-        best = stream_plan(c, c.by, rpw, cx, cus, chunk_req);
+    // Wave rows per frame at most: both kernels (equal and different
+    // shells) take the frame of the one with fewer registers to spare,
+    // except with two waves per row, where the kernel for equal shells
+    // takes 12 waves (6 wave rows, 8 core rows of 12) and the other 8 (4
+    // of 8): frames of 8 rows recompute every stored row once more.
+    const int wmax_d = stream_max_waves(cx, rpw) / wx;
+    const int wmax = wx == 2 ? sm_max_threads<8, 2, false>() / 64 / wx : wmax_d;
+    SmPlan best, bestd;
+    st_status st = stream_choose(c, rpw, cx, wx, wmax, cus, best);
+    if (st != ST_OK) return st;
+    if (wmax != wmax_d) {
+        Cfg cd = c;
+        if (!cd.heuristic && cd.by > wmax_d) cd.by = wmax_d;
+        st = stream_choose(cd, rpw, cx, wx, wmax_d, cus, bestd);
+        if (st != ST_OK) return st;
     } else {
-        // whole grid in one frame if it fits, else the frame height whose
-        // tiles fill the CUs best
-        const int wfit = (c.ny + 4 + rpw - 1) / rpw;
-        best = stream_plan(c, wfit < wmax ? (wfit < wmin ? wmin : wfit) : wmax, rpw, cx, cus, chunk_req);
-        // candidate frame heights: down to two thirds of the largest, and
-        // not below 8 waves where 8 fit (fewer waves lose the latency
-        // hiding; 512^3: 6 waves x 3 chunks ran 21% slower than 8 x 5)
-        const int wlo = std::max(std::max(wmin, (wmax * 2) / 3), std::min(wmax, 8));
-        if (wfit > wmax)
-            for (int wy = wmax - 1; wy >= wlo; --wy) {
-                const SmPlan q = stream_plan(c, wy, rpw, cx, cus, chunk_req);
-                if (q.cost < best.cost * 0.97) best = q;  // fewer waves must pay for themselves
-            }
-        // Fields of 2.5 times the 256 MiB last-level cache and more: the
-        // plan with the fewest planes marched per plane stored ((chunk + 4)
-        // / chunk) wins as long as 75-92% of the CUs hold a tile (frames of
-        // 2/3 of the largest height or more, as measured; ties go to more
-        // tiles).  At 512^3: 8 waves x 5 chunks of 103 planes (215 tiles)
-        // 9.67 ms per 100 sweeps, 10 x 8 chunks of 64 (256 tiles, the
-        // busiest-CU model's choice) 10.2, 12 x 8 of 64 (208) 9.88, with the
-        // same HBM bytes; at 448^3 8 x 6 of 75 (228 tiles) 6.72 against 6.98.
-        // At 384^3 the busiest-CU model's 256 tiles stay the fastest.
-        if (chunk_req <= 0 && 2LL * c.nx * c.ny * c.nz * 4 >= (640LL << 20)) {
-            SmPlan pick{};
-            double pick_amp = 0;
-            for (int wy = wmax; wy >= wlo; --wy) {
-                const int core_h = wy * rpw - 4;
-                const int tiles_y = (c.ny + core_h - 1) / core_h;
-                for (int nch = 1; nch * tiles_y <= cus; ++nch) {
-                    const int chunk = (c.nz + nch - 1) / nch;
-                    const int ntiles = tiles_y * ((c.nz + chunk - 1) / chunk);
-                    if (chunk < 4 || chunk > stream_max_chunk(wy, rpw, cx) || ntiles * 100 < cus * 75 ||
-                        ntiles * 100 > cus * 92)
-                        continue;
-                    const double amp = (chunk + 4.0) / chunk;
-                    if (pick.wy == 0 || amp < pick_amp - 1e-9 || (amp < pick_amp + 1e-9 && ntiles > pick.ntiles)) {
-                        pick = stream_plan(c, wy, rpw, cx, cus, chunk);
-                        pick_amp = amp;
-                    }
-                }
-            }
-            if (pick.wy) best = pick;
-        }
+        bestd = best;
     }
     c.kind = 3;
     c.cx = cx;
     c.cy = rpw;
-    c.wx = 1;
+    c.wx = wx;
     c.wy = best.wy;
     c.tsteps = 2;
-    c.lds = stream_lds(best.wy, rpw, cx, best.chunk);
-    if (c.lds > kMaxLds)
-        return fail(ST_ELDS, "streaming form: a z chunk of %d planes needs %d bytes of LDS (at most %d "
-                             "planes with this frame)", best.chunk, c.lds, stream_max_chunk(best.wy, rpw, cx));
+    c.lds = stream_lds(best.wy, rpw, cx, best.chunk, wx);
     c.bx = c.nx;
     c.by = best.core_h;
     c.zchunk = best.chunk;
     c.bz = best.chunk;
+    c.wy_d = bestd.wy;
+    c.core_d = bestd.core_h;
+    c.zchunk_d = bestd.chunk;
+    c.lds_d = stream_lds(bestd.wy, rpw, cx, bestd.chunk, wx);
     c.gx = 0;
     c.gy = c.gz = 1;
     return ST_OK;
@@ -782,35 +853,41 @@ st_status run_stream(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_check(e, "shell comparison launch");
 
-    const void *fns[2] = {stream_fn(c.op, c.cx, c.cy, false), stream_fn(c.op, c.cx, c.cy, true)};
-    SmArgs a;
-    a.nx = c.nx; a.ny = c.ny; a.nz = c.nz;
-    a.sz = (long long)c.nx * c.ny;
-    a.core_h = c.wy * c.cy - 4;
-    a.tiles_y = (c.ny + a.core_h - 1) / a.core_h;
-    a.zchunk = c.zchunk;
-    a.ntiles = a.tiles_y * ((c.nz + c.zchunk - 1) / c.zchunk);
-    a.iters = c.iters;
-    a.n1 = c.iters % 4;
-    a.n2 = (c.iters - a.n1) / 2;
-    a.shell_eq = static_cast<const int *>(ws);
-    a.xs = xs;
-    const int threads = 64 * c.wy;
-    int grid = a.ntiles;
-    for (const void *fn : fns) {
-        e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds);
+    const void *fns[2] = {stream_fn(c.op, c.cx, c.cy, false, c.wx), stream_fn(c.op, c.cx, c.cy, true, c.wx)};
+    // each kernel's frame: [0] equal shells, [1] different shells
+    SmArgs a[2];
+    int threads[2], grid[2], lds[2];
+    for (int k = 0; k < 2; ++k) {
+        const int wy = k ? c.wy_d : c.wy, chunk = k ? c.zchunk_d : c.zchunk;
+        SmArgs &b = a[k];
+        b.nx = c.nx; b.ny = c.ny; b.nz = c.nz;
+        b.sz = (long long)c.nx * c.ny;
+        b.core_h = wy * c.cy - 4;
+        b.tiles_y = (c.ny + b.core_h - 1) / b.core_h;
+        b.zchunk = chunk;
+        b.ntiles = b.tiles_y * ((c.nz + chunk - 1) / chunk);
+        b.iters = c.iters;
+        b.n1 = c.iters % 4;
+        b.n2 = (c.iters - b.n1) / 2;
+        b.shell_eq = static_cast<const int *>(ws);
+        b.xs = xs;
+        threads[k] = 64 * wy * c.wx;
+        lds[k] = k ? c.lds_d : c.lds;
+        e = hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, lds[k]);
         if (e != hipSuccess) return hip_check(e, "hipFuncSetAttribute(stream LDS)");
         int per_cu = 0;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, c.lds);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fns[k], threads[k], lds[k]);
         if (e != hipSuccess) return hip_check(e, "occupancy query");
         if (per_cu < 1) return fail(ST_ELDS, "streaming kernel does not fit on a CU");
         int dev = 0, cus = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        grid = std::min(grid, per_cu * cus);
+        grid[k] = std::min(b.ntiles, per_cu * cus);
     }
     float kf = c.op == OP_THESIS ? (float)(2 * c.dims * c.r) : (float)c.weight;
-    void *args[] = {&u1, &u2, &a, &kf};
+    void *args0[] = {&u1, &u2, &a[0], &kf};
+    void *args1[] = {&u1, &u2, &a[1], &kf};
+    void **args[2] = {args0, args1};
     // Which kernel runs: outside graph capture (and outside the no-alloc
     // entry point) the host reads the comparison flags once and launches
     // only the kernel for the case that holds; otherwise both are launched
@@ -830,11 +907,11 @@ st_status run_stream(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2
     }
     if (!c.per_pass) {
         // one cooperative launch per kernel, a grid barrier between passes
-        a.p0 = 0;
-        a.p1 = a.n2 + a.n1;
         bool coop = true;
         for (int k = k_lo; k < k_hi && coop; ++k) {
-            e = hipLaunchCooperativeKernel(fns[k], dim3(grid), dim3(threads), args, c.lds, s);
+            a[k].p0 = 0;
+            a[k].p1 = a[k].n2 + a[k].n1;
+            e = hipLaunchCooperativeKernel(fns[k], dim3(grid[k]), dim3(threads[k]), args[k], lds[k], s);
             if (e == hipErrorIllegalState && k == k_lo) {
                 // the device takes no cooperative launch from this process
                 // now: the same passes follow, one launch per pass
@@ -849,11 +926,11 @@ st_status run_stream(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2
     // one launch per pass and kernel, kernel boundaries as the barrier: n2
     // fused passes, then n1 single-sweep passes (the kernel for different
     // shells runs them too, when that is the case)
-    for (int p = 0; p < a.n2 + a.n1; ++p) {
-        a.p0 = p;
-        a.p1 = p + 1;
+    for (int p = 0; p < a[0].n2 + a[0].n1; ++p) {
         for (int k = k_lo; k < k_hi; ++k) {
-            e = hipLaunchKernel(fns[k], dim3(grid), dim3(threads), args, c.lds, s);
+            a[k].p0 = p;
+            a[k].p1 = p + 1;
+            e = hipLaunchKernel(fns[k], dim3(grid[k]), dim3(threads[k]), args[k], lds[k], s);
             if (e != hipSuccess) return hip_check(e, "stream pass launch");
         }
     }
