@@ -129,11 +129,14 @@ typedef struct st_params {
                              (F32/F64, radius 1..4, any pass length),
                              2 = register-queue full-row frames (F32, radius
                              1, temporal_steps 2, even iteration count),
-                             3 = streaming rows, one wave per grid row (F32,
-                             radius 1, nx a multiple of 4 up to 1024; two
-                             fused sweeps per pass whether or not the two
-                             buffers hold the same shell; block_y = waves
-                             per frame, coarsen_y = rows per wave); in 2D
+                             3 = streaming rows, one wave per grid row up
+                             to 512 columns and two above (F32, radius 1,
+                             nx a multiple of 4 up to 1024; two fused
+                             sweeps per pass whether or not the two buffers
+                             hold the same shell; block_y = wave rows per
+                             frame, coarsen_y = rows per wave, 1 over 512
+                             columns asks for one wave of 16 cells per
+                             lane); in 2D
                              (the only COOP form there) 512-column segments
                              per wave, six and four fused sweeps per pass
                              when both buffers hold the same shell (else

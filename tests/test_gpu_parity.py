@@ -860,7 +860,7 @@ def test_coop_cooperative_launch_timed_on_null_stream(gpu, oracle, form):
     (3, 288, 100, "smem_register"),  # below: the row kernel
     (3, 512, 2, "smem_register"),    # too few sweeps for a fused pass
     (3, 1024, 100, "coop"),          # two waves per row
-    (3, 768, 100, "smem_register"),  # a quarter of those lanes idle: the row kernel
+    (3, 768, 100, "coop"),           # the right wave of each row half idle, still ahead
     (2, 256, 64, "base"),
     (2, 1024, 64, "base"),           # launch-bound
     (2, 2048, 64, "coop"),           # above: the 2D streaming form

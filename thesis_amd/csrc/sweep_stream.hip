@@ -619,11 +619,11 @@ static int stream_cx(const Cfg &c)
     return c.nx <= 256 ? 4 : 8;
 }
 
-// AUTO's use of the form: over 512 columns a quarter of the lanes idle at
-// 768 columns, where the row kernel measured faster (768^3, one wave of 16
-// cells per row: 647 against 601 GCells/s); at 1024^3 the form gives 1010
-// against 697 with that layout.
-bool stream_auto_ok(const Cfg &c) { return stream_eligible(c) && (c.nx <= 512 || c.nx > 896); }
+// AUTO's use of the form: wherever it applies.  (With one wave of 16 cells
+// per row over 512 columns, the row kernel was faster at 768^3, 647 against
+// 601 GCells/s; with two waves of 8 per row the form gives 980 at 640^3,
+// 1120 at 768^3 and 1263 at 896^3 against 650 for the row kernel.)
+bool stream_auto_ok(const Cfg &c) { return stream_eligible(c); }
 
 bool stream_eligible(const Cfg &c)
 {
@@ -669,10 +669,15 @@ static int stream_lds(int wy, int rpw, int cx, int chunk, int wx)
     return 2 * (2 * wy * wx * (cx / 4) * 64 * 16) + 2 * wy * rpw * (chunk + 5) * 8;
 }
 
+// Dynamic LDS the form may take: the 160 KiB less the kernel's static LDS
+// (256 bytes by the compiler's resource report; hipFuncSetAttribute refused
+// a request of 163776 bytes).
+constexpr int kStreamMaxLds = kMaxLds - 256;
+
 // Deepest z chunk whose slice still fits the LDS.
 static int stream_max_chunk(int wy, int rpw, int cx, int wx)
 {
-    return (kMaxLds - stream_lds(wy, rpw, cx, -5, wx)) / (2 * wy * rpw * 8) - 5;
+    return (kStreamMaxLds - stream_lds(wy, rpw, cx, -5, wx)) / (2 * wy * rpw * 8) - 5;
 }
 
 // Tiles of one pass for `wy` waves of `rpw` rows per frame: y frames times
@@ -698,8 +703,24 @@ static SmPlan stream_plan(const Cfg &c, int wy, int rpw, int cx, int wx, int cus
         if (chunk > c.nz) chunk = c.nz;
         const int cmax = stream_max_chunk(wy, rpw, cx, wx);
         if (chunk > cmax) {
-            const int n = (c.nz + cmax - 1) / cmax;
-            chunk = (c.nz + n - 1) / n;
+            // the LDS caps the chunk: among the chunk counts from the
+            // fewest that fit, the one whose busiest CU takes the least
+            // time, as rounds of tiles x (planes per tile + 30: a tile's
+            // start and its halo planes cost about 30 planes, as fitted to
+            // 1024^3 with 128 frames of 8 core rows: 4 chunks of 256 (512
+            // tiles, 2 rounds) 1423 GCells/s, 6 of 171 (3 rounds) 1365, 8
+            // of 128 (4 rounds) 1273, 5 of 205 (2.5 rounds) 1269)
+            const int n0 = (c.nz + cmax - 1) / cmax;
+            long long best = -1;
+            for (int n = n0; n < n0 + 8 && n <= c.nz; ++n) {
+                const int ch = (c.nz + n - 1) / n;
+                const long long nt = (long long)p.tiles_y * ((c.nz + ch - 1) / ch);
+                const long long cost = (nt + cus - 1) / cus * (ch + 30LL);
+                if (best < 0 || cost < best) {
+                    best = cost;
+                    chunk = ch;
+                }
+            }
         }
     }
     p.chunk = chunk;
@@ -767,7 +788,7 @@ static st_status stream_choose(const Cfg &c, int rpw, int cx, int wx, int wmax, 
         if (pick.wy) best = pick;
     }
     const int lds = stream_lds(best.wy, rpw, cx, best.chunk, wx);
-    if (lds > kMaxLds)
+    if (lds > kStreamMaxLds)
         return fail(ST_ELDS, "streaming form: a z chunk of %d planes needs %d bytes of LDS (at most %d "
                              "planes with this frame)", best.chunk, lds, stream_max_chunk(best.wy, rpw, cx, wx));
     return ST_OK;
