@@ -156,7 +156,8 @@ def batched_leg(st, np, repeats: int = 3):
     """512 uneven 3D grids (edges log-uniform in [16, 128], radius 1 or 2),
     50 JACOBI sweeps each: one hipGraph of one launch per pass over all grids
     (two fused sweeps per pass where the shells are equal), one persistent
-    cooperative launch, and per-problem launches.  The roofline is the graph
+    cooperative launch of the same passes captured in a hipGraph, and
+    per-problem launches.  The roofline is the graph
     mode's: algorithmic bytes of one sweep of every grid (4 B x (cells +
     interior cells)) over the time per sweep."""
     from thesis_amd import workloads

@@ -272,7 +272,9 @@ typedef struct st_tile_rule {
 
 typedef enum st_batch_mode {
     ST_BATCH_GRAPH = 0,       /* one hipGraph replaying a batched launch per pass  */
-    ST_BATCH_PERSISTENT = 1,  /* one cooperative launch, grid barrier per sweep    */
+    ST_BATCH_PERSISTENT = 1,  /* one persistent cooperative launch, a grid barrier
+                                 between passes, captured in a hipGraph and
+                                 replayed                                          */
     ST_BATCH_SEQUENTIAL = 2   /* stencil_dispatch per problem (baseline)           */
 } st_batch_mode;
 
@@ -281,11 +283,18 @@ typedef enum st_batch_mode {
  * when it is NULL).  elapsed_ms (optional) covers the launches on the
  * device: in the plane-chunk form the comparison of each problem's two
  * shells (equal shells let whole rows be stored, shell cells with the value
- * they hold) and the sweeps.  ST_BATCH_GRAPH keeps the instantiated graph of
- * the last batch per device and stream and replays it when the next
- * dispatch has the same problems (buffers, shapes, iteration counts) and
- * tile rules; on the dispatch that captures and instantiates it, elapsed_ms
- * includes that host work (the stream waits meanwhile). */
+ * they hold, and let two sweeps fuse into one pass) and the sweeps.
+ * ST_BATCH_GRAPH and ST_BATCH_PERSISTENT keep the instantiated graph of the
+ * last batch per device and stream and replay it when the next dispatch
+ * has the same problems (buffers, shapes, iteration counts), tile rules and
+ * mode; on the dispatch that captures and instantiates it, elapsed_ms
+ * includes that host work (the stream waits meanwhile).  Both read the
+ * shell comparison back before launching, so neither may be called on a
+ * stream the caller is capturing, except ST_BATCH_PERSISTENT, which then
+ * launches into the caller's graph with single sweeps only; on the null
+ * stream ST_BATCH_PERSISTENT launches directly, without a graph.  After a
+ * fused pass the buffer that does not hold the result has an intermediate
+ * state in its interior (its shell is untouched). */
 st_status stencil_dispatch_batched(const st_problem *problems, int nproblems,
                                    const st_tile_rule *rules, int nrules, int mode,
                                    void *stream, float *elapsed_ms);

@@ -130,10 +130,12 @@ def test_batched_forms_edge_shapes(gpu, oracle, shapes, op):
             assert np.array_equal(bufs_got[res].view(np.uint32), bufs_want[res].view(np.uint32)), \
                 (mode, i, shapes[i], q["op"])
             # the other buffer: the previous sweep, except where the graph
-            # mode fused two sweeps per pass (equal shells, 4 sweeps or
-            # more), which leaves an intermediate state in its interior; no
-            # shell cell may change either way
-            fused = mode == 0 and np.array_equal(u1, u2) and q["iterations"] >= 4
+            # or persistent mode fused two sweeps per pass (plane form, equal
+            # shells, 4 sweeps or more), which leaves an intermediate state
+            # in its interior; no shell cell may change either way
+            plane = all((1 + 2 * r_) * nx_ <= 4096 and nx_ <= 4 * 512 // r_ for nx_, _, _, r_ in shapes) \
+                and len(set(ops)) == 1
+            fused = plane and np.array_equal(u1, u2) and q["iterations"] >= 4
             other_got, other_want = bufs_got[1 - res], bufs_want[1 - res]
             if fused:
                 shell = np.ones(other_got.shape, bool)
@@ -146,9 +148,11 @@ def test_batched_forms_edge_shapes(gpu, oracle, shapes, op):
             b.free()
 
 
+@pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("r", [1, 2])
-def test_batched_fused_passes(gpu, oracle, r):
-    """Graph mode with two sweeps fused per pass: problems with equal shells
+def test_batched_fused_passes(gpu, oracle, r, mode):
+    """Graph mode (one launch per pass) and persistent mode (one cooperative
+    launch, captured in a graph) with two sweeps fused per pass: problems with equal shells
     (every sweep count 1..9, so 0, 2 or 4 fused passes then 1-3 single
     sweeps) beside problems whose buffer 2 is zero (different shells, single
     sweeps only) in one plane-form batch; every result buffer bit-exact with
@@ -170,7 +174,7 @@ def test_batched_fused_passes(gpu, oracle, r):
             probs.append(dict(nx=nx, ny=ny, nz=nz, radius=r, iterations=its, op=1, u1=d1, u2=d2))
             host.append((u1, u2))
             bufs += [d1, d2]
-    print(f"[case] batched fused passes r={r}", flush=True)
+    print(f"[case] batched fused passes r={r} mode={mode}", flush=True)
     rules = [dict(radius=r, max_cells=10 ** 9, coarsen_y=0, zchunk=13)]
 
     def check(rounds, only_even):
@@ -183,13 +187,31 @@ def test_batched_fused_passes(gpu, oracle, r):
             assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (rounds, i, q["nx"], q["ny"], q["nz"],
                                                                                 q["iterations"])
 
-    st.dispatch_batched(probs, rules, 0, stream=stream)
+    st.dispatch_batched(probs, rules, mode, stream=stream)
     stream.sync()
     check(1, False)
     # a second dispatch replays the cached graph; it continues from the
     # results where they are in u1 (an even sweep count)
-    st.dispatch_batched(probs, rules, 0, stream=stream)
+    st.dispatch_batched(probs, rules, mode, stream=stream)
     stream.sync()
     check(2, True)
+    for b in bufs:
+        b.free()
+
+
+def test_batched_persistent_null_stream(gpu, oracle):
+    """The persistent mode on the null stream (no capture there): the
+    cooperative launch runs directly, fused passes included."""
+    st = gpu
+    shapes = [(40, 30, 20, 1), (33, 17, 19, 2), (64, 70, 23, 1)]
+    its = [6, 5, 8]
+    probs, host, bufs = _alloc(st, oracle, shapes, its, [1, 1, 1], 4000, None)
+    print("[case] batched persistent on the null stream", flush=True)
+    st.dispatch_batched(probs, [], 1)
+    st.synchronize()
+    for i, (q, (u1, u2)) in enumerate(zip(probs, host)):
+        want = oracle.run(u1.copy(), u2.copy(), 3, q["radius"], 1, q["iterations"])
+        got = (q["u2"] if q["iterations"] % 2 else q["u1"]).numpy()
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (i, shapes[i])
     for b in bufs:
         b.free()

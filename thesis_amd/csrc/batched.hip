@@ -114,8 +114,12 @@ __global__ __launch_bounds__(256) void k_batch_sweep(const BatchProblem *__restr
 
 __global__ __launch_bounds__(256) void k_batch_persistent(const BatchProblem *__restrict__ P,
                                                           const int *__restrict__ starts,
-                                                          int np, int nitems, int nsweeps)
+                                                          int np, int nitems, int nsweeps, int *coop_ok)
 {
+    // a launch that is not cooperative returns before any grid barrier and
+    // leaves coop_ok at 0 (the host reports it)
+    if (!cooperative_groups::this_grid().is_valid()) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *coop_ok = 1;
     const int wpb = blockDim.x >> 6;
     const int nw = gridDim.x * wpb;
     const int w0 = blockIdx.x * wpb + (threadIdx.x >> 6);
@@ -499,10 +503,13 @@ __device__ __forceinline__ void bp_tile2(const BatchProblem &q, int local, int p
 // then cleared by any block of k_bp_shell_eq (grid: problems x parts) that
 // finds a cell of problem p's R-thick shell whose bits differ between u1 and
 // u2.
-__global__ __launch_bounds__(256) void k_bp_flags(int *__restrict__ flags, int np)
+__global__ __launch_bounds__(256) void k_bp_flags(int *__restrict__ flags, int np, int nzero)
 {
+    // then nzero words of 0: the persistent launch's cooperative flag and
+    // its per-pass tile counters
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < np) flags[i] = 1;
+    else if (i < np + nzero) flags[i] = 0;
 }
 
 __global__ __launch_bounds__(256) void k_bp_shell_eq(const BatchProblem *__restrict__ P, int *__restrict__ flags)
@@ -553,19 +560,10 @@ __global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(OP =
     if (tile < tile_hi) bp_item<OP>(P, starts, eqf, np, tile, sweep, lds);
 }
 
-// Fused items [tile_lo, tile_hi) of pass `pass`, one workgroup each: problems
-// with pass < n2 advance two sweeps.  (THESIS's division needs more
-// registers: two waves per SIMD, one workgroup per CU.)
+// Fused tile `local` of problem q in its tile shape.
 template <int OP>
-__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(OP == OP_THESIS ? 2 : 4))) void k_bp_fused(
-    const BatchProblem *__restrict__ P, const int *__restrict__ starts, int np, int tile_lo, int tile_hi, int pass)
+__device__ __forceinline__ void bp_fused_tile(const BatchProblem &q, int local, int pass, float *lds)
 {
-    __shared__ float lds[kBp2Lds];
-    const int tile = tile_lo + (int)xcd_remap(blockIdx.x, gridDim.x);
-    if (tile >= tile_hi) return;
-    const BatchProblem &q = P[find_problem(starts, np, tile)];
-    if (pass >= q.n2) return;
-    const int local = tile - q.item0f;
     if (q.g.r == 1) {
         if (q.ncf == 8) bp_tile2<OP, 1, 8, 8>(q, local, pass, lds);
         else if (q.ncf == 4) bp_tile2<OP, 1, 4, 4>(q, local, pass, lds);
@@ -580,16 +578,62 @@ __global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(OP =
     }
 }
 
+// Fused items [tile_lo, tile_hi) of pass `pass`, one workgroup each: problems
+// with pass < n2 advance two sweeps.  (THESIS's division needs more
+// registers: two waves per SIMD, one workgroup per CU.)
 template <int OP>
-__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(OP == OP_THESIS ? 4 : 6))) void k_bp_persistent(
-    const BatchProblem *__restrict__ P, const int *__restrict__ starts, const int *__restrict__ eqf, int np,
-    int ntiles, int nsweeps)
+__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(OP == OP_THESIS ? 2 : 4))) void k_bp_fused(
+    const BatchProblem *__restrict__ P, const int *__restrict__ starts, int np, int tile_lo, int tile_hi, int pass)
 {
-    __shared__ float lds[kBpLds];
-    const int b0 = (int)xcd_remap(blockIdx.x, gridDim.x);
-    for (int s = 0; s < nsweeps; ++s) {
-        for (int tile = b0; tile < ntiles; tile += (int)gridDim.x) bp_item<OP>(P, starts, eqf, np, tile, s, lds);
-        if (s + 1 < nsweeps) cooperative_groups::this_grid().sync();
+    __shared__ float lds[kBp2Lds];
+    const int tile = tile_lo + (int)xcd_remap(blockIdx.x, gridDim.x);
+    if (tile >= tile_hi) return;
+    const BatchProblem &q = P[find_problem(starts, np, tile)];
+    if (pass >= q.n2) return;
+    bp_fused_tile<OP>(q, tile - q.item0f, pass, lds);
+}
+
+// Persistent form: one cooperative launch runs every pass of the batch, a
+// grid barrier between passes; in pass p each problem takes a fused step
+// of two sweeps (p < n2) or its single sweep p + n2 (as one launch per
+// pass does in the graph mode).  Workgroups take the pass's tiles (fused
+// ones first, the larger) from a counter per pass, as the hardware
+// dispatcher hands out a launch's workgroups; the counters start at 0
+// (k_bp_flags clears them before every launch).  passmask[p] names the
+// lists pass p runs (bit 0 fused, bit 1 single), so that a pass skips the
+// list whose tiles would all return at once.  The launch is captured
+// into a hipGraph and replayed; a launch that is not cooperative returns
+// before any barrier and leaves coop_ok at 0.
+template <int OP>
+__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(OP == OP_THESIS ? 2 : 4))) void k_bp_persistent(
+    const BatchProblem *__restrict__ P, const int *__restrict__ starts, const int *__restrict__ startsf,
+    const int *__restrict__ eqf, const int *__restrict__ passmask, int np, int ntiles, int ntilesf,
+    int npasses, int *coop_ok, int *__restrict__ next)
+{
+    __shared__ float lds[kBp2Lds > kBpLds ? kBp2Lds : kBpLds];
+    __shared__ int s_tile;
+    if (!cooperative_groups::this_grid().is_valid()) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *coop_ok = 1;
+    for (int p = 0; p < npasses; ++p) {
+        const int m = passmask[p];
+        const int nf = (m & 1) ? ntilesf : 0;
+        const int total = nf + ((m & 2) ? ntiles : 0);
+        // the first tile of every workgroup is its own index, the rest come
+        // from the counter
+        for (int tile = (int)blockIdx.x;;) {
+            if (tile >= total) break;
+            if (tile < nf) {
+                const BatchProblem &q = P[find_problem(startsf, np, tile)];
+                if (p < q.n2) bp_fused_tile<OP>(q, tile - q.item0f, p, lds);
+            } else {
+                bp_item<OP>(P, starts, eqf, np, tile - nf, p, lds);
+            }
+            if (threadIdx.x == 0) s_tile = (int)gridDim.x + atomicAdd(&next[p], 1);
+            __syncthreads();
+            tile = s_tile;
+            __syncthreads();  // read by every thread before the next grab
+        }
+        if (p + 1 < npasses) cooperative_groups::this_grid().sync();
     }
 }
 
@@ -776,13 +820,26 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
     void *ws = nullptr;
     const size_t tbytes = (table.size() * sizeof(BatchProblem) + 255) & ~(size_t)255;
     const size_t sbytes = (table.size() * sizeof(int) + 255) & ~(size_t)255;
-    const size_t fbytes = table.size() * sizeof(int);
-    std::vector<unsigned char> host(tbytes + 2 * sbytes, 0);
+    // shell flags, then the persistent launch's cooperative flag and its tile
+    // counter per pass (at most maxit passes)
+    const size_t fbytes = (table.size() + 1 + (size_t)maxit) * sizeof(int);
+    // then the lists each pass runs (bit 0: fused tiles, bit 1: single-sweep
+    // tiles), at most maxit passes
+    const size_t mbytes = ((size_t)maxit * sizeof(int) + 255) & ~(size_t)255;
+    std::vector<unsigned char> host(tbytes + 2 * sbytes + mbytes, 0);
     auto pack = [&]() {
         std::memcpy(host.data(), table.data(), table.size() * sizeof(BatchProblem));
         for (size_t i = 0; i < table.size(); ++i) {
             std::memcpy(host.data() + tbytes + i * sizeof(int), &table[i].item0, sizeof(int));
             std::memcpy(host.data() + tbytes + sbytes + i * sizeof(int), &table[i].item0f, sizeof(int));
+        }
+        for (int p = 0; p < maxit; ++p) {
+            int m = 0;
+            for (const BatchProblem &b : table) {
+                if (p < b.n2) m |= 1;
+                if (p >= b.n2 && p < b.iters - b.n2) m |= 2;
+            }
+            std::memcpy(host.data() + tbytes + 2 * sbytes + p * sizeof(int), &m, sizeof(int));
         }
     };
     pack();
@@ -794,8 +851,11 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
     const BatchProblem *P = static_cast<const BatchProblem *>(ws);
     const int *starts = reinterpret_cast<const int *>(static_cast<unsigned char *>(ws) + tbytes);
     const int *startsf = reinterpret_cast<const int *>(static_cast<unsigned char *>(ws) + tbytes + sbytes);
-    int *eqf = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + tbytes + 2 * sbytes);
+    const int *passmask = reinterpret_cast<const int *>(static_cast<unsigned char *>(ws) + tbytes + 2 * sbytes);
+    int *eqf = reinterpret_cast<int *>(static_cast<unsigned char *>(ws) + tbytes + 2 * sbytes + mbytes);
     const int *ceqf = eqf;
+    int *coop_ok = eqf + n;
+    int *next = eqf + n + 1;
     // plane form: compare the problems' shells first (on the stream, inside
     // the timed region)
     // parts per problem: about 8192 shell cells per block of the largest shell
@@ -804,9 +864,52 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
         max_shell = std::max(max_shell, shell_cells(probs[i].nx, probs[i].ny, probs[i].nz, probs[i].radius));
     const int parts = (int)std::min<long long>(256, (max_shell + 8191) / 8192);
     auto shell_check = [&]() {
-        if (!plane_form) return;
-        hipLaunchKernelGGL(k_bp_flags, dim3((n + 255) / 256), dim3(256), 0, s, eqf, n);
-        hipLaunchKernelGGL(k_bp_shell_eq, dim3(n, parts), dim3(256), 0, s, P, eqf);
+        hipLaunchKernelGGL(k_bp_flags, dim3((n + 1 + maxit + 255) / 256), dim3(256), 0, s, eqf, n, 1 + maxit);
+        if (plane_form) hipLaunchKernelGGL(k_bp_shell_eq, dim3(n, parts), dim3(256), 0, s, P, eqf);
+    };
+    // Equal shells: fused passes.  The flags are read back, and every problem
+    // whose two shells are equal takes n2 = (iters - iters mod 4)/2 fused
+    // passes of two sweeps (an even number, so the state after them is in
+    // u1 as after the same number of single sweeps), then iters mod 4 single
+    // sweeps; the others single sweeps only.  The table goes up again.
+    auto fuse_plan = [&](bool &any_fused) -> st_status {
+        any_fused = false;
+        if (!plane_form) return ST_OK;
+        std::vector<int> flags(n, 0);
+        hipError_t e = hipMemcpyAsync(flags.data(), eqf, n * sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_check(e, "batch shell flags");
+        for (int i = 0; i < n; ++i) {
+            BatchProblem &b = table[i];
+            b.n2 = (flags[i] && b.tyf > 0) ? (b.iters - b.iters % 4) / 2 : 0;
+            any_fused = any_fused || b.n2 > 0;
+        }
+        if (any_fused) {
+            pack();
+            e = hipMemcpyAsync(ws, host.data(), host.size(), hipMemcpyHostToDevice, s);
+            if (e != hipSuccess) return hip_check(e, "batch table update");
+        }
+        return ST_OK;
+    };
+    // passes: problem i runs n2 fused passes, then iters - 2*n2 single
+    // sweeps (pass index p >= n2 is its sweep p + n2)
+    auto pass_count = [&]() {
+        int passes = 0;
+        for (const BatchProblem &b : table) passes = std::max(passes, b.iters - b.n2);
+        return passes;
+    };
+    // the graph cache key: the table, the workspace, the launch shape and mode
+    auto graph_key = [&]() {
+        std::vector<unsigned char> key(host.size() + sizeof(void *) + 8 * sizeof(int));
+        unsigned char *k = key.data();
+        std::memcpy(k, host.data(), host.size());
+        k += host.size();
+        std::memcpy(k, &ws, sizeof(void *));
+        k += sizeof(void *);
+        const int groups_ = plane_form ? nitems : (nitems + 3) / 4;
+        const int meta[8] = {n, nitems, nitemsf, maxit, (int)plane_form, (int)thesis, groups_, mode};
+        std::memcpy(k, meta, sizeof meta);
+        return key;
     };
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (elapsed_ms) {
@@ -817,6 +920,13 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
     // tiles of the plane-chunk form
     const int groups = plane_form ? nitems : (nitems + 3) / 4;
     if (mode == ST_BATCH_PERSISTENT) {
+        // One persistent cooperative launch (fused passes where the shells
+        // are equal, a grid barrier between passes), captured into a
+        // hipGraph that is kept for the next dispatch with the same table,
+        // workspace and shape, and replayed.  The null stream takes the
+        // cooperative launch directly; a stream that the caller is capturing
+        // takes it into the caller's graph, with single sweeps only (the
+        // shell flags cannot be read back there).
         const void *fn = !plane_form ? reinterpret_cast<const void *>(&k_batch_persistent)
                          : thesis    ? reinterpret_cast<const void *>(&k_bp_persistent<OP_THESIS>)
                                      : reinterpret_cast<const void *>(&k_bp_persistent<OP_JACOBI>);
@@ -826,16 +936,71 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
         if (e != hipSuccess) return hip_check(e, "occupancy");
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        int grid = per_cu * cus;
-        if (grid > groups) grid = groups;
-        if (grid < 1) grid = 1;
-        int np = n, ni = nitems, ns = maxit;
-        void *args_bp[] = {&P, &starts, &ceqf, &np, &ni, &ns};
-        void *args_march[] = {&P, &starts, &np, &ni, &ns};
-        if (elapsed_ms) (void)hipEventRecord(e0, s);
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        const bool in_capture = s != nullptr && hipStreamIsCapturing(s, &cs) == hipSuccess &&
+                                cs != hipStreamCaptureStatusNone;
+        if (elapsed_ms && !in_capture) (void)hipEventRecord(e0, s);
         shell_check();
-        e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), plane_form ? args_bp : args_march, 0, s);
-        if (e != hipSuccess) return hip_check(e, "batched cooperative launch");
+        bool any_fused = false;
+        if (!in_capture && (st = fuse_plan(any_fused)) != ST_OK) return st;
+        int grid = per_cu * cus;
+        grid = std::min(grid, std::max(groups, plane_form ? nitemsf : 0));
+        if (grid < 1) grid = 1;
+        int np = n, ni = nitems, nif = nitemsf, npass = plane_form ? pass_count() : maxit;
+        void *args_bp[] = {&P, &starts, &startsf, &ceqf, &passmask, &np, &ni, &nif, &npass, &coop_ok, &next};
+        void *args_march[] = {&P, &starts, &np, &ni, &npass, &coop_ok};
+        void **args = plane_form ? args_bp : args_march;
+        if (in_capture) {
+            if (elapsed_ms) {  // nothing runs until the caller's graph does
+                *elapsed_ms = 0.0f;
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+            }
+            e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, s);
+            return hip_check(e, "batched persistent launch");
+        }
+        const bool can_capture = s != nullptr;
+        std::vector<unsigned char> key = graph_key();
+        std::lock_guard<std::mutex> lock(g_graph_mutex);
+        BatchGraph &cached = g_graphs[std::make_pair(dev, s)];
+        if (can_capture && (!cached.exec || cached.key != key)) {
+            if (cached.exec) {
+                (void)hipGraphExecDestroy(cached.exec);
+                cached.exec = nullptr;
+                cached.key.clear();
+            }
+            hipGraph_t graph = nullptr;
+            e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) return hip_check(e, "begin capture");
+            e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, s);
+            hipError_t ee = hipStreamEndCapture(s, &graph);
+            if (e != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return hip_check(e, "batched cooperative launch (capture)");
+            }
+            if (ee != hipSuccess) return hip_check(ee, "end capture");
+            hipGraphExec_t exec = nullptr;
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return hip_check(e, "graph instantiate");
+            cached.exec = exec;
+            cached.key = std::move(key);
+        }
+        e = can_capture ? hipGraphLaunch(cached.exec, s)
+                        : hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, s);
+        if (e != hipSuccess) return hip_check(e, "batched persistent launch");
+        if (elapsed_ms) (void)hipEventRecord(e1, s);
+        int ok = 0;
+        e = hipMemcpyAsync(&ok, coop_ok, sizeof ok, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_check(e, "batched persistent launch");
+        if (ok != 1) return fail(ST_ELAUNCH, "the persistent batch launch ran without a cooperative grid");
+        if (elapsed_ms) {
+            (void)hipEventElapsedTime(elapsed_ms, e0, e1);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+        return ST_OK;
     } else {
         // Graph mode.  The shells are compared first and the flags read back,
         // so that every problem with equal shells takes fused passes of two
@@ -848,34 +1013,12 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
         bool any_fused = false;
         if (plane_form) {
             shell_check();
-            std::vector<int> flags(n, 0);
-            hipError_t e = hipMemcpyAsync(flags.data(), eqf, fbytes, hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess) e = hipStreamSynchronize(s);
-            if (e != hipSuccess) return hip_check(e, "batch shell flags");
-            for (int i = 0; i < n; ++i) {
-                BatchProblem &b = table[i];
-                b.n2 = (flags[i] && b.tyf > 0) ? (b.iters - b.iters % 4) / 2 : 0;
-                any_fused = any_fused || b.n2 > 0;
-            }
-            if (any_fused) {
-                pack();
-                e = hipMemcpyAsync(ws, host.data(), tbytes, hipMemcpyHostToDevice, s);
-                if (e != hipSuccess) return hip_check(e, "batch table update");
-            }
+            if ((st = fuse_plan(any_fused)) != ST_OK) return st;
         }
         const dim3 block(plane_form ? kBpThreads : 256);
         int dev = 0;
         (void)hipGetDevice(&dev);
-        std::vector<unsigned char> key(host.size() + sizeof(void *) + 7 * sizeof(int));
-        {
-            unsigned char *k = key.data();
-            std::memcpy(k, host.data(), host.size());
-            k += host.size();
-            std::memcpy(k, &ws, sizeof(void *));
-            k += sizeof(void *);
-            const int meta[7] = {n, nitems, nitemsf, maxit, (int)plane_form, (int)thesis, groups};
-            std::memcpy(k, meta, sizeof meta);
-        }
+        std::vector<unsigned char> key = graph_key();
         std::lock_guard<std::mutex> lock(g_graph_mutex);
         BatchGraph &cached = g_graphs[std::make_pair(dev, s)];
         hipError_t e = hipSuccess;
@@ -888,10 +1031,7 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
             hipGraph_t graph = nullptr;
             e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
             if (e != hipSuccess) return hip_check(e, "begin capture");
-            // passes: problem i runs n2 fused passes, then iters - 2*n2
-            // single sweeps (pass index p >= n2 is its sweep p + n2)
-            int passes = 0;
-            for (const BatchProblem &b : table) passes = std::max(passes, b.iters - b.n2);
+            const int passes = pass_count();
             for (int p = 0; p < passes; ++p) {
                 bool fused = false, single = false;
                 for (const BatchProblem &b : table) {
