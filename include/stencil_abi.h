@@ -288,11 +288,11 @@ typedef enum st_batch_mode {
  * last batch per device and stream and replay it when the next dispatch
  * has the same problems (buffers, shapes, iteration counts), tile rules and
  * mode; on the dispatch that captures and instantiates it, elapsed_ms
- * includes that host work (the stream waits meanwhile).  Both read the
- * shell comparison back before launching, so neither may be called on a
- * stream the caller is capturing, except ST_BATCH_PERSISTENT, which then
- * launches into the caller's graph with single sweeps only; on the null
- * stream ST_BATCH_PERSISTENT launches directly, without a graph.  After a
+ * includes that host work (the stream waits meanwhile).  No mode may be
+ * called on a stream under capture (ST_EINVAL): the table goes up from
+ * host memory and the shell comparison is read back before the launches.
+ * On the null stream ST_BATCH_PERSISTENT launches directly, without a
+ * graph.  After a
  * fused pass the buffer that does not hold the result has an intermediate
  * state in its interior (its shell is untouched). */
 st_status stencil_dispatch_batched(const st_problem *problems, int nproblems,

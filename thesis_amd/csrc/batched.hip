@@ -808,6 +808,15 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
         }
         return ST_OK;
     }
+    {
+        // the graph and persistent modes upload the problem table from host
+        // memory and read the shell comparison back, then capture and replay
+        // their own graph: none of that can be recorded into a caller's
+        // capture
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (s != nullptr && hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+            return fail(ST_EINVAL, "batched dispatch on a stream under capture (it captures its own graph)");
+    }
     bool plane_form = true;
     for (int i = 0; i < n; ++i) plane_form = plane_form && bp_fits(probs[i]) && probs[i].op == probs[0].op;
     const bool thesis = probs[0].op == ST_OP_THESIS;
@@ -924,9 +933,7 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
         // are equal, a grid barrier between passes), captured into a
         // hipGraph that is kept for the next dispatch with the same table,
         // workspace and shape, and replayed.  The null stream takes the
-        // cooperative launch directly; a stream that the caller is capturing
-        // takes it into the caller's graph, with single sweeps only (the
-        // shell flags cannot be read back there).
+        // cooperative launch directly.
         const void *fn = !plane_form ? reinterpret_cast<const void *>(&k_batch_persistent)
                          : thesis    ? reinterpret_cast<const void *>(&k_bp_persistent<OP_THESIS>)
                                      : reinterpret_cast<const void *>(&k_bp_persistent<OP_JACOBI>);
@@ -936,13 +943,10 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
         if (e != hipSuccess) return hip_check(e, "occupancy");
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        const bool in_capture = s != nullptr && hipStreamIsCapturing(s, &cs) == hipSuccess &&
-                                cs != hipStreamCaptureStatusNone;
-        if (elapsed_ms && !in_capture) (void)hipEventRecord(e0, s);
+        if (elapsed_ms) (void)hipEventRecord(e0, s);
         shell_check();
         bool any_fused = false;
-        if (!in_capture && (st = fuse_plan(any_fused)) != ST_OK) return st;
+        if ((st = fuse_plan(any_fused)) != ST_OK) return st;
         int grid = per_cu * cus;
         grid = std::min(grid, std::max(groups, plane_form ? nitemsf : 0));
         if (grid < 1) grid = 1;
@@ -950,15 +954,6 @@ st_status run_batched(const st_problem *probs, int n, const st_tile_rule *rules,
         void *args_bp[] = {&P, &starts, &startsf, &ceqf, &passmask, &np, &ni, &nif, &npass, &coop_ok, &next};
         void *args_march[] = {&P, &starts, &np, &ni, &npass, &coop_ok};
         void **args = plane_form ? args_bp : args_march;
-        if (in_capture) {
-            if (elapsed_ms) {  // nothing runs until the caller's graph does
-                *elapsed_ms = 0.0f;
-                (void)hipEventDestroy(e0);
-                (void)hipEventDestroy(e1);
-            }
-            e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, s);
-            return hip_check(e, "batched persistent launch");
-        }
         const bool can_capture = s != nullptr;
         std::vector<unsigned char> key = graph_key();
         std::lock_guard<std::mutex> lock(g_graph_mutex);
