@@ -312,8 +312,8 @@ def test_full_size_512_bench_schedule(gpu, oracle, init):
     """The benchmark's own schedule at its own size: AUTO at 512^3 R=1 runs
     the streaming form, one launch per fused pass of two sweeps (103-plane z
     chunks, 215 tiles), checked against the oracle after 8 sweeps.  With the
-    reference's initial condition (buffer 2 zero, different shells) every
-    fused pass is followed by the band fix-up."""
+    reference's initial condition (buffer 2 zero, different shells) the
+    fused passes take each level's shell values from its own buffer."""
     st = gpu
     n, its = 512, 8
     op = st.OP_JACOBI if init == "seeded" else st.OP_THESIS
@@ -338,6 +338,40 @@ def test_full_size_512_bench_schedule(gpu, oracle, init):
         u2 = np.zeros_like(u1)
     want = oracle.run(u1, u2, 3, 1, op, its)
     assert_bitwise(got, want, f"512^3 bench schedule {init}")
+
+
+@pytest.mark.parametrize("init", ["seeded", "reference"])
+def test_full_size_1024_stream(gpu, oracle, init):
+    """The reference's 3D evaluation size: AUTO at 1024^3 R=1 runs the
+    streaming form with two waves per row (12-wave frames with equal
+    shells, 8-wave frames with different ones, z chunks set by the LDS),
+    checked against the oracle after 6 sweeps (two fused passes and two
+    single ones)."""
+    st = gpu
+    n, its = 1024, 6
+    op = st.OP_JACOBI if init == "seeded" else st.OP_THESIS
+    p = st.StencilParams.make(dim=n, radius=1, iterations=its, op=op, variant="auto", heuristic=1)
+    cfg = st.launch_config(p)
+    assert cfg["variant"] == st.V_COOP and (cfg["coarsen_x"], cfg["coarsen_y"]) == (8, 2), cfg
+    _announce("1024^3 stream", init, cfg)
+    d1 = st.DeviceArray(p.shape, np.float32)
+    d2 = st.DeviceArray(p.shape, np.float32)
+    st.initialize_region(p, d1, d2, st.INIT_SEEDED if init == "seeded" else st.INIT_REFERENCE,
+                         seed=0x5EED0002)
+    in_u2, _ = st.dispatch_kernels(p, d1, d2)
+    assert not in_u2
+    got = d1.numpy()
+    d1.free()
+    d2.free()
+    if init == "seeded":
+        u1 = oracle.init_seeded(n, n, n, 3, 1, 0x5EED0002)
+        u2 = u1.copy()
+    else:
+        u1 = oracle.init_reference(n, n, n, 3)
+        u2 = np.zeros_like(u1)
+    want = oracle.run(u1, u2, 3, 1, op, its)
+    del u1, u2
+    assert_bitwise(got, want, f"1024^3 stream {init}")
 
 
 def test_full_size_mixed_384(gpu, oracle):
