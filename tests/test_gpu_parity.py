@@ -374,6 +374,40 @@ def test_full_size_1024_stream(gpu, oracle, init):
     assert_bitwise(got, want, f"1024^3 stream {init}")
 
 
+@pytest.mark.parametrize("init", ["seeded", "reference"])
+def test_full_size_2d_32768(gpu, oracle, init):
+    """The reference's 2D evaluation size at its 8 sweeps: AUTO at 32768^2
+    R=1 runs the 2D streaming form (two passes of four fused sweeps with
+    equal shells; single sweeps with different ones), checked against the
+    oracle."""
+    st = gpu
+    n, its = 32768, 8
+    op = st.OP_JACOBI if init == "seeded" else st.OP_THESIS
+    p = st.StencilParams.make(nx=n, ny=n, nz=1, dimensions=2, radius=1, iterations=its, op=op,
+                              variant="auto", heuristic=1)
+    cfg = st.launch_config(p)
+    assert cfg["variant"] == st.V_COOP, cfg
+    _announce("32768^2 stream2d", init, cfg)
+    d1 = st.DeviceArray(p.shape, np.float32)
+    d2 = st.DeviceArray(p.shape, np.float32)
+    st.initialize_region(p, d1, d2, st.INIT_SEEDED if init == "seeded" else st.INIT_REFERENCE,
+                         seed=0x5EED0001)
+    in_u2, _ = st.dispatch_kernels(p, d1, d2)
+    assert not in_u2
+    got = d1.numpy()
+    d1.free()
+    d2.free()
+    if init == "seeded":
+        u1 = oracle.init_seeded(n, n, 1, 2, 1, 0x5EED0001)
+        u2 = u1.copy()
+    else:
+        u1 = oracle.init_reference(n, n, 1, 2)
+        u2 = np.zeros_like(u1)
+    want = oracle.run(u1, u2, 2, 1, op, its)
+    del u1, u2
+    assert_bitwise(got, want, f"32768^2 stream2d {init}")
+
+
 def test_full_size_mixed_384(gpu, oracle):
     """The 384^3 R=4 workload at full size.  (1) Its 16-bit kernels (two
     waves per row) bit-exact with the oracle over 23 sweeps (two defect
