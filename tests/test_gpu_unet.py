@@ -92,6 +92,33 @@ def test_conv3_backward_data_and_weights(gpu):
     assert rel(gb.numpy(), gb_ref) < 1e-5
 
 
+@pytest.mark.parametrize("shape,ci,co", [((2, 4, 5, 16), 16, 32), ((1, 3, 4, 8), 96, 32), ((1, 4, 4, 64), 8, 16),
+                                         ((2, 3, 3, 24), 48, 64), ((1, 2, 3, 40), 32, 16)])
+def test_conv3_wgrad_channel_major(gpu, shape, ci, co):
+    """Rows of a multiple of 8 positions take the channel-major weight
+    gradient (transposed operands, 16-byte loads, taps with dx = +-1 joined
+    from two aligned loads), against the FP64 oracle."""
+    st = gpu
+    from thesis_amd import nn
+    rng = np.random.default_rng(ci + 7 * co)
+    N, D, H, W = shape
+    P = N * D * H * W
+    x = U.bf16_round(rng.standard_normal(shape + (ci,)))
+    w = np.zeros((co, nn.kpad(ci)))
+    w[:, :27 * ci] = U.bf16_round(rng.standard_normal((co, 27 * ci)) * 0.2)
+    gy = U.bf16_round(rng.standard_normal(shape + (co,)))
+    _, gw_ref, _ = U.conv3_bwd(x, w[:, :27 * ci], gy)
+    dx = st.DeviceArray.from_numpy(bits(x).reshape(P, ci))
+    dg = st.DeviceArray.from_numpy(bits(gy).reshape(P, co))
+    gw = st.DeviceArray((co, nn.kpad(ci)), np.float32)
+    gw.fill_zero()
+    print(f"[case] wgrad channel-major {shape} ci={ci} co={co}", flush=True)
+    nn.conv3_wgrad(dx, dg, gw, shape, ci, co)
+    g = gw.numpy()
+    assert rel(g[:, :27 * ci], gw_ref) < 1e-5, (shape, ci, co)
+    assert not g[:, 27 * ci:].any()
+
+
 def test_small_layers(gpu):
     st = gpu
     from thesis_amd import nn

@@ -270,6 +270,130 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(const bf16_t *__restrict__ 
     }
 }
 
+// ---------------------------------------------------------------- weight gradient, channel-major operands
+
+// The MFMA operands of the weight gradient run along positions (8
+// consecutive positions of one channel per lane), which channels-last
+// activations hold Ci values apart: the gather above issues one 2-byte load
+// per value.  When rows are a multiple of 8 positions, both operands are
+// first transposed to channel-major [C][P] copies, and then every operand is
+// one 16-byte load: the dY rows directly, the X rows of a tap with dx = -1
+// or +1 from two aligned loads joined by a 16-bit funnel shift (a group of 8
+// positions never crosses a row, so only its first or last value can fall
+// outside the row, and it is zeroed).
+
+// [P][C] -> [C][P] (16-bit words) through LDS, 64 positions per workgroup.
+constexpr int kTrMaxC = 128;
+__global__ __launch_bounds__(256) void k_pc_transpose(const bf16_t *__restrict__ src, bf16_t *__restrict__ dst,
+                                                      int P, int C)
+{
+    __shared__ bf16_t t[kTrMaxC * 66];
+    const long long p0 = (long long)blockIdx.x * 64;
+    const int np = (int)min(64LL, (long long)P - p0);
+    for (int i = threadIdx.x; i < np * C; i += 256) {
+        const int pp = i / C, c = i - pp * C;
+        t[c * 66 + pp] = src[p0 * C + i];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 64 * C; i += 256) {
+        const int c = i >> 6, pp = i & 63;
+        if (pp < np) dst[(long long)c * P + p0 + pp] = t[c * 66 + pp];
+    }
+}
+
+__device__ __forceinline__ u32x4 load8u(const bf16_t *p) { return *reinterpret_cast<const u32x4 *>(p); }
+
+// dW[co][k] += sum_p dYt[co][p] * Xt[ci(k)][p + off(tap(k))]; the same
+// tiling as k_conv3_wgrad (rows = output channels, 16*NB weight columns,
+// 32 positions per MFMA, `pchunk` positions per wave, atomics at the end).
+// Requires W % 8 == 0 and pchunk % 32 == 0.
+template <int NB>
+__global__ __launch_bounds__(256) void k_conv3_wgrad_t(const bf16_t *__restrict__ Xt,
+                                                       const bf16_t *__restrict__ dYt,
+                                                       float *__restrict__ dW, ConvGeom g, int pchunk)
+{
+    const int lane = threadIdx.x & 63;
+    const int j = lane & 15, grp = lane >> 4;
+    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int K = 27 * g.Ci;
+    const int nkt = (K + 16 * NB - 1) / (16 * NB);
+    const int nct = (g.Co + 15) / 16;
+    const long long nchunks = ((long long)g.P + pchunk - 1) / pchunk;
+    const long long total = nchunks * nkt * nct;
+    if (wave >= total) return;
+    const int kt = (int)(wave % nkt);
+    const long long r2 = wave / nkt;
+    const int ct = (int)(r2 % nct);
+    const int pc = (int)(r2 / nct);
+    const int co = ct * 16 + j;  // A-operand row of this lane
+    const long long HW = (long long)g.H * g.W;
+    const bf16_t *arow = dYt + (long long)min(co, g.Co - 1) * g.P;
+    int kk[NB], dz[NB], dy[NB], dx[NB];
+    const bf16_t *brow[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+        kk[nb] = kt * 16 * NB + nb * 16 + j;  // B-operand column of this lane
+        const int kc = kk[nb] < K ? kk[nb] : 0;
+        const int tap = kc / g.Ci;
+        const int ci = kc - tap * g.Ci;
+        dz[nb] = tap / 9 - 1;
+        dy[nb] = (tap / 3) % 3 - 1;
+        dx[nb] = tap % 3 - 1;
+        brow[nb] = Xt + (long long)ci * g.P;
+    }
+    f32x4 acc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int pbeg = pc * pchunk, pend = min(pbeg + pchunk, g.P);
+    const u32x4 zero = {0u, 0u, 0u, 0u};
+    for (int ps = pbeg; ps < pend; ps += 32) {
+        // this lane group's 8 positions: one row segment (W % 8 == 0)
+        const int pf = ps + grp * 8;
+        const bool pin = pf < pend;
+        int n0, z0, y0, x0;
+        decode(g, min(pf, g.P - 8), n0, z0, y0, x0);
+        const u32x4 av = (pin && co < g.Co) ? load8u(arow + pf) : zero;
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const int zz = z0 + dz[nb], yy = y0 + dy[nb];
+            const bool ok = pin && kk[nb] < K && zz >= 0 && zz < g.D && yy >= 0 && yy < g.H;
+            u32x4 b = zero;
+            if (ok) {
+                const bf16_t *q = brow[nb] + pf + (long long)dz[nb] * HW + (long long)dy[nb] * g.W;
+                const u32x4 v = load8u(q);
+                if (dx[nb] == 0) {
+                    b = v;
+                } else if (dx[nb] < 0) {
+                    // values x0-1 .. x0+6: the last of the group before, then v[0..6]
+                    const unsigned prev = x0 > 0 ? load8u(q - 8)[3] : 0u;
+                    b[0] = __builtin_amdgcn_alignbit(v[0], prev, 16);
+                    b[1] = __builtin_amdgcn_alignbit(v[1], v[0], 16);
+                    b[2] = __builtin_amdgcn_alignbit(v[2], v[1], 16);
+                    b[3] = __builtin_amdgcn_alignbit(v[3], v[2], 16);
+                } else {
+                    // values x0+1 .. x0+8: v[1..7], then the first of the group after
+                    const unsigned next = x0 + 8 < g.W ? load8u(q + 8)[0] : 0u;
+                    b[0] = __builtin_amdgcn_alignbit(v[1], v[0], 16);
+                    b[1] = __builtin_amdgcn_alignbit(v[2], v[1], 16);
+                    b[2] = __builtin_amdgcn_alignbit(v[3], v[2], 16);
+                    b[3] = __builtin_amdgcn_alignbit(next, v[3], 16);
+                }
+            }
+            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av),
+                                                              __builtin_bit_cast(bf16x8, b), acc[nb], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+        if (kk[nb] >= K) continue;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int c = ct * 16 + 4 * grp + e;
+            if (c < g.Co) atomicAdd(dW + (long long)c * g.Kpad + kk[nb], acc[nb][e]);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- small layers
 
 // W[Co][Kpad] (k = tap*Ci + ci)  ->  Wt[Ci][KpadT] (k' = tap'*Co + co) with
@@ -723,8 +847,22 @@ st_status st_nn_conv3_wgrad(const void *x, const void *dy, float *dw, int N, int
     const int K = 27 * Ci;
     const long long nkt = (K + 63) / 64, nct = (Co + 15) / 16, nch = (g.P + pchunk - 1) / pchunk;
     const long long waves = nkt * nct * nch;
-    hipLaunchKernelGGL((k_conv3_wgrad<4>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0,
-                       static_cast<hipStream_t>(stream), static_cast<const bf16_t *>(x),
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const dim3 grid((unsigned)((waves + 3) / 4));
+    if (W % 8 == 0 && Ci <= kTrMaxC && Co <= kTrMaxC) {
+        // channel-major copies of both operands in the workspace (slot 0)
+        void *ws = nullptr;
+        st_status st = workspace_slot(0, ((size_t)g.P * Ci + (size_t)g.P * Co) * sizeof(bf16_t), s, &ws);
+        if (st != ST_OK) return st;
+        bf16_t *xt = static_cast<bf16_t *>(ws);
+        bf16_t *dyt = xt + (size_t)g.P * Ci;
+        const dim3 tg((unsigned)((g.P + 63) / 64));
+        hipLaunchKernelGGL(k_pc_transpose, tg, dim3(256), 0, s, static_cast<const bf16_t *>(x), xt, g.P, Ci);
+        hipLaunchKernelGGL(k_pc_transpose, tg, dim3(256), 0, s, static_cast<const bf16_t *>(dy), dyt, g.P, Co);
+        hipLaunchKernelGGL((k_conv3_wgrad_t<4>), grid, dim3(256), 0, s, xt, dyt, dw, g, pchunk);
+        return hip_check(hipGetLastError(), "conv3 wgrad launch");
+    }
+    hipLaunchKernelGGL((k_conv3_wgrad<4>), grid, dim3(256), 0, s, static_cast<const bf16_t *>(x),
                        static_cast<const bf16_t *>(dy), dw, g, pchunk);
     return hip_check(hipGetLastError(), "conv3 wgrad launch");
 }
