@@ -677,11 +677,14 @@ static int bp_rows(const st_problem &p, int want)
 // Planes per tile of the fused form: its chunks carry 4R halo planes (2R in
 // the single form), so they are deeper than the tuned single-form chunks
 // (mostly 16 planes).  Measured on the 512-grid batch (50 sweeps, graph
-// mode): the tuned chunks 7.09 ms, whole grids in z 7.38, 64 planes 6.64,
-// 32 planes 6.49.
+// mode), with one tile shape for all: the tuned chunks 7.09 ms, whole grids
+// in z 7.38, 64 planes 6.64, 32 planes 6.49; with per-problem tile shapes
+// (three runs each on one box): 32 planes 6.02-6.20, 48 for R = 2 only
+// 5.95-6.01, 48 for both radii 5.95 (48 for R = 1 only 6.37-6.40, 40 for
+// both 6.15, 64 for R = 1 6.83).
 static int fused_chunk(const st_problem &p)
 {
-    const int want = 32;
+    const int want = 48;
     return want > p.nz ? p.nz : want;
 }
 
