@@ -27,7 +27,10 @@ int st_nn_kpad(int ci);
 st_status st_nn_conv3(const void *act, const void *w, const float *bias, void *out_bf16,
                       float *out_f32, int N, int D, int H, int W, int Ci, int Co, int relu,
                       int accumulate, void *stream);
-/* dW[co][k] += sum_p dY[p][co] X[p+off(tap)][ci]  (FP32, atomic accumulation) */
+/* dW[co][k] += sum_p dY[p][co] X[p+off(tap)][ci]  (FP32, atomic accumulation).
+ * With W a multiple of 8 (and Ci, Co <= 128) both operands are first copied
+ * channel-major into the library's workspace (P*(Ci+Co) 16-bit words, kept
+ * per device and stream). */
 st_status st_nn_conv3_wgrad(const void *x, const void *dy, float *dw, int N, int D, int H, int W,
                             int Ci, int Co, void *stream);
 /* backward-data weights: Wt[ci][tap'*Co + co] = W[co][(26 - tap')*Ci + ci] */
