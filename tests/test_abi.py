@@ -150,3 +150,30 @@ def test_coop_row_form_config(st):
     assert f4["launches"] == 1 and f4["temporal_steps"] == 1
     _raises(st, 5, variant="coop", temporal_form=4, nx=62, ny=40, nz=40, heuristic=1)
     _raises(st, 1, variant="coop", temporal_form=5, dim=64, heuristic=1)
+
+
+def test_streaming_form_plans(st):
+    """The streaming COOP form's frame and chunk plans (host logic, no GPU):
+    the 512^3 benchmark's 8-wave frames of 16 rows (12 stored) in 103-plane
+    chunks; over 512 columns two waves of 8 cells per row and 12-row frames
+    (8 stored), the chunk count searched when the LDS caps the chunk; one
+    wave of 16 cells per row on request; LDS within 160 KiB less the
+    kernel's own 256 bytes."""
+    def plan(n, **kw):
+        p = st.StencilParams.make(dim=n, radius=1, iterations=100, op=st.OP_JACOBI, **kw)
+        return st.launch_config(p)
+    c = plan(512, variant="auto", heuristic=1)
+    assert c["variant"] == st.VARIANTS["coop"]
+    assert (c["coarsen_x"], c["coarsen_y"], c["block_y"], c["zchunk"]) == (8, 2, 12, 103), c
+    for n, chunk in ((1024, 256), (896, 224), (768, 154), (1000, 250)):
+        c = plan(n, variant="auto", heuristic=1)
+        assert c["variant"] == st.VARIANTS["coop"], (n, c)
+        assert (c["coarsen_x"], c["coarsen_y"], c["block_y"], c["zchunk"]) == (8, 2, 8, chunk), (n, c)
+        assert c["lds_bytes"] <= 160 * 1024 - 256, (n, c)
+    one = plan(1024, variant="coop", temporal_form=3, heuristic=0, block_x=1024, block_y=8, block_z=64,
+               coarsen_y=1)
+    assert (one["coarsen_x"], one["coarsen_y"], one["block_y"]) == (16, 1, 4), one
+    # a chunk whose LDS slice does not fit is refused up front (ST_ELDS)
+    with pytest.raises(st.StencilError) as e:
+        plan(1024, variant="coop", temporal_form=3, heuristic=0, block_x=1024, block_y=6, block_z=400)
+    assert e.value.status == 2, str(e.value)

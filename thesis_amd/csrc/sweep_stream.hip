@@ -732,18 +732,10 @@ static SmPlan stream_plan(const Cfg &c, int wy, int rpw, int cx, int wx, int cus
     return p;
 }
 
-// Frame and chunk of one kernel of the form (at most wmax wave rows).
-static st_status stream_choose(const Cfg &c, int rpw, int cx, int wx, int wmax, int cus, SmPlan &best)
+// The library's frame and chunk for one kernel (at most wmax wave rows).
+static void stream_choose_auto(const Cfg &c, int rpw, int cx, int wx, int wmax, int cus, int chunk_req, int wmin,
+                               SmPlan &best)
 {
-    const int chunk_req = c.zchunk > 0 ? c.zchunk : 0;
-    const int wmin = (4 + rpw) / rpw;  // at least one core row
-    if (!c.heuristic && c.by > 0) {
-        // block_y = wave rows per frame (frame of rpw*block_y rows)
-        if (c.by < wmin || c.by > wmax)
-            return fail(ST_EINVAL, "streaming form: block_y (wave rows per frame) must be %d..%d", wmin, wmax);
-        best = stream_plan(c, c.by, rpw, cx, wx, cus, chunk_req);
-        return ST_OK;
-    }
     // whole grid in one frame if it fits, else the frame height whose
     // tiles fill the CUs best
     const int wfit = (c.ny + 4 + rpw - 1) / rpw;
@@ -786,6 +778,21 @@ static st_status stream_choose(const Cfg &c, int rpw, int cx, int wx, int wmax, 
             }
         }
         if (pick.wy) best = pick;
+    }
+}
+
+// Frame and chunk of one kernel of the form (at most wmax wave rows).
+static st_status stream_choose(const Cfg &c, int rpw, int cx, int wx, int wmax, int cus, SmPlan &best)
+{
+    const int chunk_req = c.zchunk > 0 ? c.zchunk : 0;
+    const int wmin = (4 + rpw) / rpw;  // at least one core row
+    if (!c.heuristic && c.by > 0) {
+        // block_y = wave rows per frame (frame of rpw*block_y rows)
+        if (c.by < wmin || c.by > wmax)
+            return fail(ST_EINVAL, "streaming form: block_y (wave rows per frame) must be %d..%d", wmin, wmax);
+        best = stream_plan(c, c.by, rpw, cx, wx, cus, chunk_req);
+    } else {
+        stream_choose_auto(c, rpw, cx, wx, wmax, cus, chunk_req, wmin, best);
     }
     const int lds = stream_lds(best.wy, rpw, cx, best.chunk, wx);
     if (lds > kStreamMaxLds)
