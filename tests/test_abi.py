@@ -177,3 +177,23 @@ def test_streaming_form_plans(st):
     with pytest.raises(st.StencilError) as e:
         plan(1024, variant="coop", temporal_form=3, heuristic=0, block_x=1024, block_y=6, block_z=400)
     assert e.value.status == 2, str(e.value)
+
+
+def test_stream2d_schedules(st):
+    """The 2D streaming form's pass schedule (host logic, no GPU): passes of
+    six and four fused sweeps, an even number of them (so the state ends
+    where single sweeps leave it), as few as possible and then as deep as
+    possible; under 8 sweeps AUTO keeps a single-sweep kernel."""
+    def cfg(its, **kw):
+        kw.setdefault("variant", "auto")
+        p = st.StencilParams.make(nx=32768, ny=32768, nz=1, dimensions=2, radius=1, iterations=its,
+                                  op=st.OP_JACOBI, heuristic=1, **kw)
+        return st.launch_config(p)
+    # 8 = 4 + 4, 12 = 6 + 6, 64 = 8 x 6 + 4 x 4, 24 = 4 x 6
+    for its, launches in ((8, 2), (12, 2), (64, 12), (24, 4)):
+        c = cfg(its)
+        assert c["variant"] == st.VARIANTS["coop"] and c["launches"] == launches, (its, c)
+        assert (c["block_x"], c["coarsen_x"]) == (512, 8), c
+    assert cfg(7)["variant"] != st.VARIANTS["coop"]
+    # passes of four only on request: 64 = 16 x 4
+    assert cfg(64, variant="coop", temporal_steps=4)["launches"] == 16
