@@ -146,8 +146,31 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
     // prefetch storage: 16-bit inputs as packed words
     typedef typename std::conditional<MODE == RM_CORR, uint32_t, T>::type P;
     constexpr int NP = MODE == RM_CORR ? CX / 2 : CX;
+    // LDS quads: an FP64 quad is kept as two 16-byte halves in separate
+    // 64-lane arrays, so every 16-byte access of a wave is contiguous (with
+    // the 32-byte lane stride of whole quads each access was a two-way bank
+    // conflict)
+    constexpr int NH = (int)sizeof(QV) / 16;
+    typedef T HQ __attribute__((ext_vector_type(4 / NH)));
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    QV *lds = reinterpret_cast<QV *>(lds_raw);
+    HQ *lds = reinterpret_cast<HQ *>(lds_raw);
+    auto lds_put = [](HQ *b, int qi, int l, const T *v) {
+#pragma unroll
+        for (int hh = 0; hh < NH; ++hh) {
+            HQ x;
+#pragma unroll
+            for (int j = 0; j < 4 / NH; ++j) x[j] = v[hh * (4 / NH) + j];
+            b[(qi * NH + hh) * 64 + l] = x;
+        }
+    };
+    auto lds_get = [](const HQ *b, int qi, int l, T *v) {
+#pragma unroll
+        for (int hh = 0; hh < NH; ++hh) {
+            const HQ x = b[(qi * NH + hh) * 64 + l];
+#pragma unroll
+            for (int j = 0; j < 4 / NH; ++j) v[hh * (4 / NH) + j] = x[j];
+        }
+    };
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // 8 cells per lane are laid out only for rows of one wave, so the seam
@@ -284,26 +307,16 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
             __builtin_amdgcn_sched_barrier(0);
             T nw[CX];
             widen_row(pf, nw);
-            QV *buf = lds + (size_t)((m & 1) * lrows) * QW * 64;
+            HQ *buf = lds + (size_t)((m & 1) * lrows) * QW * NH * 64;
 #pragma unroll
-            for (int h = 0; h < H; ++h) {
-                QV q;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) q[j] = nw[4 * h + j];
-                buf[((R + wr) * QW + wxp * H + h) * 64 + lane] = q;
-            }
+            for (int h = 0; h < H; ++h) lds_put(buf, (R + wr) * QW + wxp * H + h, lane, nw + 4 * h);
 #pragma unroll
             for (int i = 0; i < HS; ++i) {
                 if (!has_halo[i]) continue;
                 T hv[CX];
                 widen_row(ph[i], hv);
 #pragma unroll
-                for (int h = 0; h < H; ++h) {
-                    QV q;
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) q[j] = hv[4 * h + j];
-                    buf[(hrow[i] * QW + hp[i] * H + h) * 64 + lane] = q;
-                }
+                for (int h = 0; h < H; ++h) lds_put(buf, hrow[i] * QW + hp[i] * H + h, lane, hv + 4 * h);
             }
             const int k = m - R;
             const bool out_plane = store_row && k >= k_out_lo && k < k_out_hi;
@@ -341,8 +354,9 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
                 rq[j] = nq[H - 1][j];
             }
             if (wx > 1) {
-                const QV le = buf[((R + wr) * QW + max(wxp * H - 1, 0)) * 64 + 63];
-                const QV re = buf[((R + wr) * QW + min(wxp * H + H, QW - 1)) * 64];
+                T le[4], re[4];
+                lds_get(buf, (R + wr) * QW + max(wxp * H - 1, 0), 63, le);
+                lds_get(buf, (R + wr) * QW + min(wxp * H + H, QW - 1), 0, re);
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const T l0 = le[j], r0 = re[j];
@@ -388,7 +402,8 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
             for (int d = R; d >= 1; --d) {
 #pragma unroll
                 for (int h = 0; h < H; ++h) {
-                    const QV q = buf[((R + wr - d) * QW + wxp * H + h) * 64 + lane];
+                    T q[4];
+                    lds_get(buf, (R + wr - d) * QW + wxp * H + h, lane, q);
 #pragma unroll
                     for (int j = 0; j < 4; ++j) s[4 * h + j] += q[j];
                 }
@@ -397,7 +412,8 @@ __global__ __launch_bounds__((rows_max_threads<R, CX, T, MODE, HS>())) void k_ro
             for (int d = 1; d <= R; ++d) {
 #pragma unroll
                 for (int h = 0; h < H; ++h) {
-                    const QV q = buf[((R + wr + d) * QW + wxp * H + h) * 64 + lane];
+                    T q[4];
+                    lds_get(buf, (R + wr + d) * QW + wxp * H + h, lane, q);
 #pragma unroll
                     for (int j = 0; j < 4; ++j) s[4 * h + j] += q[j];
                 }
