@@ -137,7 +137,7 @@ typedef struct st_params {
                              frame, coarsen_y = rows per wave, 1 over 512
                              columns asks for one wave of 16 cells per
                              lane); in 2D
-                             (the only COOP form there) 512-column segments
+                             (with the grid-stride form below) 512-column segments
                              per wave, six and four fused sweeps per pass
                              when both buffers hold the same shell (else
                              single sweeps; temporal_steps 4 keeps passes of
@@ -145,7 +145,13 @@ typedef struct st_params {
                              chunk,
                              4 = row frames of the SMEM_REGISTER row form, one
                              sweep per grid barrier (3D F32/F64, radius 1..4,
-                             nx a multiple of 4).
+                             nx a multiple of 4),
+                             5 = grid-stride cells, one sweep per grid
+                             barrier (the reference's coop kernel,
+                             src/stencil_kernel_coop.cu:5-49: 2D/3D,
+                             F32/F64, any radius; the library's choice for
+                             radius 5 and up and for 2D grids the streaming
+                             form does not take).
                              SMEM_REGISTER: 0 = library's choice, 1 = register
                              march, 2 = row streaming (3D F32/F64, radius
                              1..4, nx a multiple of 4; block_y = rows per

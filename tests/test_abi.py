@@ -85,9 +85,11 @@ def test_invalid_knobs(st):
     _raises(st, 2, variant="smem_padded", block_x=16, block_y=8, block_z=8, radius=8, dim=64,
             coarsen_x=4)                         # LDS over 160 KiB
     _raises(st, 5, variant="smem_register", radius=17, dim=64, heuristic=1)
-    _raises(st, 5, variant="coop", radius=5, dim=64, heuristic=1)
-    _raises(st, 5, variant="coop", dimensions=2, dim=64, radius=2, heuristic=1)  # 2D streaming: radius 1
-    _raises(st, 5, variant="coop", dimensions=2, dim=60, heuristic=1)            # nx not a multiple of 8
+    _raises(st, 5, variant="coop", radius=5, dim=64, heuristic=1, temporal_form=1)  # LDS rings: radius 1..4
+    _raises(st, 5, variant="coop", dimensions=2, dim=64, radius=2, heuristic=1,
+            temporal_form=3)                                                   # 2D streaming: radius 1
+    _raises(st, 5, variant="coop", dimensions=2, dim=60, heuristic=1,
+            temporal_form=3)                                                   # nx not a multiple of 8
 
 
 def test_launch_config_shapes(st):
@@ -149,7 +151,7 @@ def test_coop_row_form_config(st):
                                                 temporal_form=4, heuristic=1))
     assert f4["launches"] == 1 and f4["temporal_steps"] == 1
     _raises(st, 5, variant="coop", temporal_form=4, nx=62, ny=40, nz=40, heuristic=1)
-    _raises(st, 1, variant="coop", temporal_form=5, dim=64, heuristic=1)
+    _raises(st, 1, variant="coop", temporal_form=6, dim=64, heuristic=1)
 
 
 def test_streaming_form_plans(st):
@@ -177,6 +179,35 @@ def test_streaming_form_plans(st):
     with pytest.raises(st.StencilError) as e:
         plan(1024, variant="coop", temporal_form=3, heuristic=0, block_x=1024, block_y=6, block_z=400)
     assert e.value.status == 2, str(e.value)
+
+
+def test_coop_grid_stride_plans(st):
+    """COOP at radius 5 and up (3D) and on 2D grids the streaming form does
+    not take runs the grid-stride form (the reference's coop kernel): one
+    sweep per grid barrier, one cooperative launch or one launch per sweep;
+    the LDS and register forms refuse radius 5 and up (host logic, no GPU)."""
+    def cfg(**kw):
+        kw.setdefault("variant", "coop")
+        kw.setdefault("heuristic", 1)
+        return st.launch_config(st.StencilParams.make(**kw))
+    for r in (5, 8, 16, 32, 128):
+        c = cfg(dim=300, radius=r, iterations=3)
+        assert (c["variant"], c["temporal_steps"], c["launches"], c["lds_bytes"]) == \
+            (st.VARIANTS["coop"], 1, 1, 0), (r, c)
+    assert cfg(dim=40, radius=8, iterations=3, launch_per_pass=1)["launches"] == 3
+    assert cfg(dim=40, radius=2, iterations=4, temporal_form=5)["temporal_steps"] == 1
+    # 2D: nx not a multiple of 8, FP64 or radius 2 leave the streaming form
+    for kw in (dict(nx=131, ny=77), dict(nx=128, ny=77, dtype=st.F64), dict(nx=128, ny=77, radius=2)):
+        kw.setdefault("radius", 1)
+        c = cfg(nz=1, dimensions=2, iterations=10, **kw)
+        assert (c["temporal_steps"], c["launches"]) == (1, 1), (kw, c)
+    assert cfg(nx=128, ny=77, nz=1, dimensions=2, radius=1, iterations=10)["temporal_steps"] > 1
+    for form in (1, 2, 3, 4):
+        with pytest.raises(st.StencilError) as e:
+            cfg(dim=40, radius=8, iterations=3, temporal_form=form)
+        assert "grid-stride" in str(e.value) or e.value.status != 0, str(e.value)
+    with pytest.raises(st.StencilError):
+        cfg(dim=40, radius=1, iterations=3, temporal_form=6)
 
 
 def test_stream2d_schedules(st):

@@ -87,7 +87,7 @@ CASES = [
 
 def _supported(variant, dims, r, dtype):
     if variant == "coop":
-        return dims == 3 and r <= 4
+        return True  # radius 5 and up, and 2D off the streaming form: the grid-stride form
     if variant == "smem_padded" and dims == 3 and r > 8:
         return False  # the (b+2R)^3 tile exceeds 160 KiB of LDS (test_padded_r16_3d_rejected)
     if variant == "smem_register":
@@ -115,6 +115,42 @@ def test_variant_vs_oracle(gpu, oracle, variant, case):
     got, in_u2 = run_gpu(st, p, u1, u2)
     assert in_u2 == bool(its % 2)
     assert_bitwise(got, want, f"{variant} {case}")
+
+
+# COOP's grid-stride form (temporal_form 5; the library's choice for radius
+# 5 and up): (dims, nx, ny, nz, radius, op, dtype, iterations, form,
+# launch_per_pass); the 200x190x180 grid has more interior cells than the
+# co-resident threads, so every thread strides
+GS_CASES = [
+    (3, 41, 37, 35, 5, 1, "f32", 5, 0, 1),
+    (3, 40, 30, 28, 2, 0, "f64", 4, 5, 0),
+    (3, 80, 72, 70, 32, 1, "f32", 2, 0, 0),
+    (3, 200, 190, 180, 6, 1, "f32", 3, 0, 0),
+    (3, 200, 190, 180, 6, 0, "f64", 2, 0, 1),
+    (2, 256, 96, 1, 1, 0, "f32", 6, 5, 0),
+    (2, 300, 65, 1, 3, 1, "f64", 5, 0, 0),
+]
+
+
+@pytest.mark.parametrize("case", GS_CASES, ids=lambda c: "d%d_%dx%dx%d_r%d_op%d_%s_it%d_form%d_pp%d" % c)
+def test_coop_grid_stride_vs_oracle(gpu, oracle, case):
+    st = gpu
+    dims, nx, ny, nz, r, op, dt, its, form, per_pass = case
+    _announce("coop grid-stride", case)
+    npdt = np.float64 if dt == "f64" else np.float32
+    u1 = oracle.init_seeded(nx, ny, nz, dims, r, 0x5EED0100 + nx, dtype=npdt)
+    rng = np.random.default_rng(nx * 11 + r)
+    u1 = np.where(u1 == 0, rng.random(u1.shape), u1).astype(npdt)
+    u2 = u1.copy()
+    want = oracle.run(u1.copy(), u2.copy(), dims, r, op, its).copy()
+    p = st.StencilParams.make(nx=nx, ny=ny, nz=nz, dimensions=dims, radius=r, iterations=its, op=op,
+                              dtype=st.F64 if dt == "f64" else st.F32, variant="coop", heuristic=1,
+                              temporal_form=form, launch_per_pass=per_pass)
+    cfg = st.launch_config(p)
+    assert cfg["temporal_steps"] == 1 and cfg["launches"] == (its if per_pass else 1), cfg
+    got, in_u2 = run_gpu(st, p, u1, u2)
+    assert in_u2 == bool(its % 2)
+    assert_bitwise(got, want, f"coop grid-stride {case}")
 
 
 # explicit (non-heuristic) shapes, including the reference's own defaults

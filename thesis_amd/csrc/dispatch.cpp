@@ -127,7 +127,7 @@ static st_status validate_common(const st_params *p, bool mixed)
     }
     if (p->coarsen_x < 0 || p->coarsen_y < 0 || p->smem_pad < 0 || p->temporal_steps < 0 ||
         p->launch_per_pass < 0 || p->launch_per_pass > 1 || p->temporal_form < 0 ||
-        p->temporal_form > 4)
+        p->temporal_form > 5)
         return fail(ST_EINVAL, "coarsening, padding, temporal steps, launch_per_pass or "
                                "temporal_form out of range");
     const bool heur = p->heuristic || p->block_x <= 0;
@@ -312,7 +312,7 @@ st_status stencil_launch_config(const st_params *p, st_launch_info *info)
             // streaming form per pass: n2 fused passes, then iters mod 4
             // single ones
             if (c.kind == 3) n = c.iters > 0 ? (c.per_pass ? (c.iters - c.iters % 4) / 2 + c.iters % 4 : 1) : 0;
-            if (c.kind == 5) n = c.iters;
+            if (c.kind == 5 || c.kind == 7) n = c.iters;
             info->launches = n ? (c.per_pass ? n : 1) : 0;
             // 2D streaming form, always one launch per pass (with equal
             // shells: fused passes of six and four sweeps, then single sweeps)

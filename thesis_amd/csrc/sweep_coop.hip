@@ -31,6 +31,8 @@
 // above.
 #include <hip/hip_cooperative_groups.h>
 
+#include <algorithm>
+
 #include "st_device.hpp"
 #include "st_internal.hpp"
 
@@ -210,16 +212,135 @@ static long long coop_lds(const Cfg &c, int t)
 
 // For COOP, block_x/block_y are the output tile (TX, TY), block_z the plane
 // chunk, coarsen_y is unused and the workgroup is 256 threads.
+// ---------------------------------------------------------------- grid-stride form
+//
+// The reference's coop kernel as designed (src/stencil_kernel_coop.cu:5-49):
+// every thread steps over the interior cells, x fastest, with the grid's
+// stride, and a grid barrier separates the sweeps; the per-sweep ping-pong
+// (sweep s reads buffer s%2) and the sum order of the other kernels (x, y,
+// then z terms, nearest last on the minus side, nearest first on the plus
+// side).  Any radius and either dimensionality: the COOP form for radius 5
+// and up, where the LDS-ring, register and row forms are not built, and for
+// 2D grids the streaming form does not take.  Sweeps [s0, s1) of the run.
+template <int OP, int D, typename T>
+__global__ __launch_bounds__(256) void k_coop_gs(T *__restrict__ u1, T *__restrict__ u2, Geom g, int s0, int s1,
+                                                 T kc)
+{
+    const int R = g.r;
+    const int nxi = g.nx - 2 * R, nyi = g.ny - 2 * R, nzi = D == 3 ? g.nz - 2 * R : 1;
+    const long long n = (nxi > 0 && nyi > 0 && nzi > 0) ? (long long)nxi * nyi * nzi : 0;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (int s = s0; s < s1; ++s) {
+        if (s > s0) cooperative_groups::this_grid().sync();
+        const T *in = (s & 1) ? u2 : u1;
+        T *out = (s & 1) ? u1 : u2;
+        for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
+            const long long t = c / nxi;
+            const int i = R + (int)(c - t * nxi);
+            const int j = R + (int)(t % nyi);
+            const int k = D == 3 ? R + (int)(t / nyi) : 0;
+            const long long idx = i + (long long)j * g.sy + (long long)k * g.sz;
+            T u = 0;
+            for (int d = R; d >= 1; --d) u += in[idx - d];
+            for (int d = 1; d <= R; ++d) u += in[idx + d];
+            for (int d = R; d >= 1; --d) u += in[idx - d * g.sy];
+            for (int d = 1; d <= R; ++d) u += in[idx + d * g.sy];
+            if constexpr (D == 3) {
+                for (int d = R; d >= 1; --d) u += in[idx - d * g.sz];
+                for (int d = 1; d <= R; ++d) u += in[idx + d * g.sz];
+            }
+            out[idx] = finish<OP>(u, in[idx], kc);
+        }
+    }
+}
+
+template <typename T>
+static const void *coop_gs_fn(int op, int dims)
+{
+    if (dims == 3)
+        return op == OP_THESIS ? reinterpret_cast<const void *>(&k_coop_gs<OP_THESIS, 3, T>)
+                               : reinterpret_cast<const void *>(&k_coop_gs<OP_JACOBI, 3, T>);
+    return op == OP_THESIS ? reinterpret_cast<const void *>(&k_coop_gs<OP_THESIS, 2, T>)
+                           : reinterpret_cast<const void *>(&k_coop_gs<OP_JACOBI, 2, T>);
+}
+
+static st_status resolve_coop_gs(Cfg &c)
+{
+    c.kind = 7;
+    c.tsteps = 1;
+    c.bx = 256;
+    c.by = c.bz = 1;
+    c.cx = c.cy = 1;
+    c.zchunk = 0;
+    c.lds = 0;
+    c.gx = 0;  // decided at launch from occupancy
+    c.gy = c.gz = 1;
+    return ST_OK;
+}
+
+// Every sweep in one cooperative launch with a grid barrier between sweeps
+// (the grid capped at the co-resident workgroups), or one launch per sweep
+// with launch_per_pass or when the device refuses the cooperative launch.
+static st_status run_coop_gs(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2)
+{
+    *in_u2 = c.iters & 1;
+    if (c.iters <= 0) return ST_OK;
+    const void *fn = c.dtype == ST_F64 ? coop_gs_fn<double>(c.op, c.dims) : coop_gs_fn<float>(c.op, c.dims);
+    Geom g;
+    g.nx = c.nx; g.ny = c.ny; g.nz = c.dims == 3 ? c.nz : 1; g.r = c.r;
+    g.sy = c.nx; g.sz = (long long)c.nx * c.ny;
+    const long long cells = (long long)std::max(c.nx - 2 * c.r, 0) * std::max(c.ny - 2 * c.r, 0) *
+                            (c.dims == 3 ? std::max(c.nz - 2 * c.r, 0) : 1);
+    int per_cu = 0, dev = 0, cus = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0);
+    if (e != hipSuccess) return hip_check(e, "occupancy query (grid-stride coop)");
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return hip_check(e, "hipGetDevice");
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return hip_check(e, "CU count");
+    if (per_cu < 1 || cus < 1) return fail(ST_ELAUNCH, "grid-stride COOP kernel does not fit on a CU");
+    long long grid = (cells + 255) / 256;
+    if (grid > (long long)per_cu * cus) grid = (long long)per_cu * cus;
+    if (grid < 1) grid = 1;
+    double kd = c.op == OP_THESIS ? (double)(2 * c.dims * c.r) : c.weight;
+    float kf = (float)kd;
+    void *karg = c.dtype == ST_F64 ? static_cast<void *>(&kd) : static_cast<void *>(&kf);
+    auto launch = [&](int s0, int s1, bool coop) {
+        void *args[] = {&u1, &u2, &g, &s0, &s1, karg};
+        return coop ? hipLaunchCooperativeKernel(fn, dim3((unsigned)grid), dim3(256), args, 0, s)
+                    : hipLaunchKernel(fn, dim3((unsigned)grid), dim3(256), args, 0, s);
+    };
+    if (!c.per_pass && c.iters > 1) {
+        e = launch(0, c.iters, true);
+        if (e == hipSuccess) return ST_OK;
+        if (e != hipErrorIllegalState) return hip_check(e, "hipLaunchCooperativeKernel(grid-stride coop)");
+        // the device takes no cooperative launch from this process now: the
+        // same sweeps, one launch each
+        (void)hipGetLastError();
+    }
+    for (int i = 0; i < c.iters; ++i)
+        if ((e = launch(i, i + 1, false)) != hipSuccess) return hip_check(e, "grid-stride coop launch");
+    return ST_OK;
+}
+
 st_status resolve_coop(Cfg &c)
 {
+    if (c.form == 5) return resolve_coop_gs(c);
     if (c.dims == 2) {
-        // 2D: the streaming form (four fused sweeps per pass)
+        // 2D: the streaming form (four or six fused sweeps per pass) where
+        // it applies, else the grid-stride form
         if (c.form != 0 && c.form != 3)
-            return fail(ST_EUNSUPPORTED, "COOP in 2D has the streaming form (temporal_form 0 or 3) only");
+            return fail(ST_EUNSUPPORTED, "COOP in 2D has the streaming and grid-stride forms (temporal_form 0, 3 "
+                                         "or 5)");
+        if (c.form == 0 && !stream2d_eligible(c)) return resolve_coop_gs(c);
         return resolve_stream2d(c);
     }
     if (c.dims != 3) return fail(ST_EUNSUPPORTED, "COOP is implemented for 2D and 3D grids");
-    if (c.r > 4) return fail(ST_EUNSUPPORTED, "COOP is built for radius 1..4");
+    if (c.r > 4) {
+        if (c.form != 0)
+            return fail(ST_EUNSUPPORTED, "COOP forms 1-4 are built for radius 1..4; radius %d takes the "
+                                         "grid-stride form (temporal_form 0 or 5)", c.r);
+        return resolve_coop_gs(c);
+    }
     const bool heur = c.heuristic || c.bx <= 0;
     // streaming form: requested, or the library's choice for heuristic
     // shapes where it applies (two fused sweeps)
@@ -304,6 +425,7 @@ st_status run_coop(const Cfg &c, void *u1, void *u2, hipStream_t s, int *in_u2)
     if (c.kind == 2) return run_tblock(c, u1, u2, s, in_u2);
     if (c.kind == 3) return run_stream(c, u1, u2, s, in_u2);
     if (c.kind == 5) return run_rows_coop(c, u1, u2, s, in_u2);
+    if (c.kind == 7) return run_coop_gs(c, u1, u2, s, in_u2);
     if (c.kind == 6) {
         // sweeps outside fused passes: the library's per-sweep 2D kernel
         Cfg sc = c;
