@@ -214,42 +214,53 @@ static long long coop_lds(const Cfg &c, int t)
 // chunk, coarsen_y is unused and the workgroup is 256 threads.
 // ---------------------------------------------------------------- grid-stride form
 //
-// The reference's coop kernel as designed (src/stencil_kernel_coop.cu:5-49):
-// every thread steps over the interior cells, x fastest, with the grid's
-// stride, and a grid barrier separates the sweeps; the per-sweep ping-pong
-// (sweep s reads buffer s%2) and the sum order of the other kernels (x, y,
-// then z terms, nearest last on the minus side, nearest first on the plus
-// side).  Any radius and either dimensionality: the COOP form for radius 5
-// and up, where the LDS-ring, register and row forms are not built, and for
-// 2D grids the streaming form does not take.  Sweeps [s0, s1) of the run.
+// The reference's coop kernel (src/stencil_kernel_coop.cu:5-49): persistent
+// workgroups stride over the interior, a grid barrier between the sweeps.
+// The unit of the stride is a group of four interior rows of one plane (one
+// wave per row, its lanes 64 columns apart along the row, as the plain
+// kernel's 64x4 blocks).  It is a coverage form, not a fast one: 29.5
+// GCells/s at 512^3 R=8 against 107 for the plain kernel (31.6 with single
+// cells as the unit, 23.6 with contiguous row ranges per workgroup).  The
+// per-sweep ping-pong (sweep s reads buffer s%2) and the sum
+// order of the other kernels (x, y, then z terms, nearest last on the minus
+// side, nearest first on the plus side).  Any radius and either
+// dimensionality: the COOP form for radius 5 and up, where the LDS-ring,
+// register and row forms are not built, and for 2D grids the streaming form
+// does not take.  Sweeps [s0, s1) of the run.
+constexpr int kGsRows = 4;
+
 template <int OP, int D, typename T>
-__global__ __launch_bounds__(256) void k_coop_gs(T *__restrict__ u1, T *__restrict__ u2, Geom g, int s0, int s1,
-                                                 T kc)
+__global__ __launch_bounds__(64 * kGsRows) void k_coop_gs(T *__restrict__ u1, T *__restrict__ u2, Geom g, int s0,
+                                                          int s1, T kc)
 {
     const int R = g.r;
     const int nxi = g.nx - 2 * R, nyi = g.ny - 2 * R, nzi = D == 3 ? g.nz - 2 * R : 1;
-    const long long n = (nxi > 0 && nyi > 0 && nzi > 0) ? (long long)nxi * nyi * nzi : 0;
-    const long long stride = (long long)gridDim.x * blockDim.x;
+    const int ngy = nyi > 0 ? (nyi + kGsRows - 1) / kGsRows : 0;
+    const long long ngroups = (nxi > 0 && nyi > 0 && nzi > 0) ? (long long)ngy * nzi : 0;
+    const int lane = threadIdx.x & 63, wr = threadIdx.x >> 6;
     for (int s = s0; s < s1; ++s) {
         if (s > s0) cooperative_groups::this_grid().sync();
         const T *in = (s & 1) ? u2 : u1;
         T *out = (s & 1) ? u1 : u2;
-        for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
-            const long long t = c / nxi;
-            const int i = R + (int)(c - t * nxi);
-            const int j = R + (int)(t % nyi);
-            const int k = D == 3 ? R + (int)(t / nyi) : 0;
-            const long long idx = i + (long long)j * g.sy + (long long)k * g.sz;
-            T u = 0;
-            for (int d = R; d >= 1; --d) u += in[idx - d];
-            for (int d = 1; d <= R; ++d) u += in[idx + d];
-            for (int d = R; d >= 1; --d) u += in[idx - d * g.sy];
-            for (int d = 1; d <= R; ++d) u += in[idx + d * g.sy];
-            if constexpr (D == 3) {
-                for (int d = R; d >= 1; --d) u += in[idx - d * g.sz];
-                for (int d = 1; d <= R; ++d) u += in[idx + d * g.sz];
+        for (long long q = blockIdx.x; q < ngroups; q += gridDim.x) {
+            const int gy = (int)(q % ngy);
+            const int j = R + gy * kGsRows + wr;
+            const int k = D == 3 ? R + (int)(q / ngy) : 0;
+            if (j >= g.ny - R) continue;
+            const long long row = (long long)j * g.sy + (long long)k * g.sz;
+            for (int i = R + lane; i < g.nx - R; i += 64) {
+                const long long idx = row + i;
+                T u = 0;
+                for (int d = R; d >= 1; --d) u += in[idx - d];
+                for (int d = 1; d <= R; ++d) u += in[idx + d];
+                for (int d = R; d >= 1; --d) u += in[idx - d * g.sy];
+                for (int d = 1; d <= R; ++d) u += in[idx + d * g.sy];
+                if constexpr (D == 3) {
+                    for (int d = R; d >= 1; --d) u += in[idx - d * g.sz];
+                    for (int d = 1; d <= R; ++d) u += in[idx + d * g.sz];
+                }
+                out[idx] = finish<OP>(u, in[idx], kc);
             }
-            out[idx] = finish<OP>(u, in[idx], kc);
         }
     }
 }
@@ -268,8 +279,9 @@ static st_status resolve_coop_gs(Cfg &c)
 {
     c.kind = 7;
     c.tsteps = 1;
-    c.bx = 256;
-    c.by = c.bz = 1;
+    c.bx = 64;
+    c.by = kGsRows;
+    c.bz = 1;
     c.cx = c.cy = 1;
     c.zchunk = 0;
     c.lds = 0;
@@ -292,13 +304,16 @@ static st_status run_coop_gs(const Cfg &c, void *u1, void *u2, hipStream_t s, in
     const long long cells = (long long)std::max(c.nx - 2 * c.r, 0) * std::max(c.ny - 2 * c.r, 0) *
                             (c.dims == 3 ? std::max(c.nz - 2 * c.r, 0) : 1);
     int per_cu = 0, dev = 0, cus = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * kGsRows, 0);
     if (e != hipSuccess) return hip_check(e, "occupancy query (grid-stride coop)");
     if ((e = hipGetDevice(&dev)) != hipSuccess) return hip_check(e, "hipGetDevice");
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return hip_check(e, "CU count");
     if (per_cu < 1 || cus < 1) return fail(ST_ELAUNCH, "grid-stride COOP kernel does not fit on a CU");
-    long long grid = (cells + 255) / 256;
+    // one workgroup per group of kGsRows interior rows, at most the co-resident ones
+    long long grid = cells > 0 ? (long long)((c.ny - 2 * c.r + kGsRows - 1) / kGsRows) *
+                                     (c.dims == 3 ? c.nz - 2 * c.r : 1)
+                               : 0;
     if (grid > (long long)per_cu * cus) grid = (long long)per_cu * cus;
     if (grid < 1) grid = 1;
     double kd = c.op == OP_THESIS ? (double)(2 * c.dims * c.r) : c.weight;
@@ -306,8 +321,8 @@ static st_status run_coop_gs(const Cfg &c, void *u1, void *u2, hipStream_t s, in
     void *karg = c.dtype == ST_F64 ? static_cast<void *>(&kd) : static_cast<void *>(&kf);
     auto launch = [&](int s0, int s1, bool coop) {
         void *args[] = {&u1, &u2, &g, &s0, &s1, karg};
-        return coop ? hipLaunchCooperativeKernel(fn, dim3((unsigned)grid), dim3(256), args, 0, s)
-                    : hipLaunchKernel(fn, dim3((unsigned)grid), dim3(256), args, 0, s);
+        return coop ? hipLaunchCooperativeKernel(fn, dim3((unsigned)grid), dim3(64 * kGsRows), args, 0, s)
+                    : hipLaunchKernel(fn, dim3((unsigned)grid), dim3(64 * kGsRows), args, 0, s);
     };
     if (!c.per_pass && c.iters > 1) {
         e = launch(0, c.iters, true);
