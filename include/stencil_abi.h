@@ -70,7 +70,9 @@ typedef enum st_op { ST_OP_THESIS = 0, ST_OP_JACOBI = 1 } st_op;
  *                                                  y in 2D), cross-lane x
  *   COOP          coop (src/stencil_kernel_coop.cu) persistent cooperative
  *                                                  kernel, grid-synchronised,
- *                                                  temporal blocking
+ *                                                  temporal blocking (radius
+ *                                                  1..4; grid-stride sweeps
+ *                                                  beyond)
  *   AUTO          fastest family for the shape (library's own choice)      */
 typedef enum st_variant {
     ST_V_BASE = 0,
